@@ -1,0 +1,167 @@
+/*
+ * rtq: ray queries on one AMD Instinct MI355X (gfx950), C ABI.
+ *
+ * The entry points replace the three surfaces through which the reference renderer feeds and
+ * runs its traversal kernels:
+ *   upload-BVH / upload-triangles  BVHData::build -> IRenderer::CreateBuffer + UploadBuffer
+ *                                  (Renderer/BVHData.cpp:28-48, rtDevice/public/IRenderer.h:30-33)
+ *                                  and BVHGeometry::flush (Renderer/BVHGeometry.cpp:69-78)
+ *   dispatch-rays                  RayTracingPass::tracePass -> IRenderContext::Dispatch
+ *                                  (Renderer/raytracingPass.cpp:89-110, rtDevice/public/IRenderContext.h:70)
+ *   read-hits                      the reference keeps hits in a GPU-only buffer
+ *                                  (Renderer/raytracingPass.cpp:72-74); reading them back is new API.
+ *
+ * Conventions: every call returns rtq_status and never aborts; handles are opaque; the caller owns
+ * host memory, the library owns the device copies it makes; ray/hit/mask buffers passed to
+ * dispatch calls are device pointers owned by the caller; `stream` is a hipStream_t (NULL is the
+ * null stream) and dispatch calls are asynchronous on it.
+ */
+#ifndef RTQ_H
+#define RTQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rtq_status {
+  RTQ_OK = 0,
+  RTQ_ERR_INVALID_ARGUMENT = 1,
+  RTQ_ERR_HIP = 2,
+  RTQ_ERR_OUT_OF_MEMORY = 3,
+  RTQ_ERR_LIMIT = 4,    /* input beyond a documented limit (tree depth, counts, capacities) */
+  RTQ_ERR_BAD_BLOB = 5, /* blob and ranges are inconsistent */
+  RTQ_ERR_NO_DEVICE = 6,
+  RTQ_ERR_BUDGET = 7,     /* serving: a request can never fit the HBM budget */
+  RTQ_ERR_QUEUE_FULL = 8, /* serving: admission queue full, retry later */
+  RTQ_ERR_CANCELLED = 9,
+  RTQ_ERR_TIMEOUT = 10,
+  RTQ_ERR_INTERNAL = 11
+} rtq_status;
+
+/* One sub-range of the flattened BVH blob: byte offset and byte size. */
+typedef struct rtq_range {
+  uint32_t offset;
+  uint32_t size;
+} rtq_range;
+
+/* The seven ranges BVHBuilder::fillGpuBuffer returns, in its argument order
+ * (Renderer/BVHBuilder.h:101; bound to shader slots by BVHData::fillBvhBindings, BVHData.cpp:17-26). */
+typedef struct rtq_ranges {
+  rtq_range triangles, primitives, materials, lights, nodes, leaf_data, blas;
+} rtq_ranges;
+
+/* Ray record, 32 B: the reference Ray {from, dir} (Shaders/core/primitive_cpp.glsl:28-35) plus the
+ * traversal's t limit, which the reference passes separately (TMAX for primary rays,
+ * baseRaytracingPass.glsl:18; the light distance for shadow rays, bvhLighting.glsl:122). */
+typedef struct rtq_ray {
+  float org[3];
+  float tmax;
+  float dir[3];
+  uint32_t pad;
+} rtq_ray;
+
+/* Closest-hit record, 16 B. tri_id is the builder insertion index of the accepted triangle
+ * (RTQ_NO_HIT on a miss); t is the value the reference stores, t_hit - 1e-5f
+ * (bvhCollision.glsl:210), or the ray's tmax on a miss; u, v are the Moller-Trumbore
+ * barycentrics of the accepted test (collision.glsl:79,84), 0 on a miss. */
+typedef struct rtq_hit {
+  uint32_t tri_id;
+  float t;
+  float u;
+  float v;
+} rtq_hit;
+
+#define RTQ_NO_HIT 0xFFFFFFFFu
+
+/* Frustum of a pinhole camera as the reference's PassData carries it (Shaders/struct_cpp.glsl:10-28,
+ * filled by RayTracingPass::fillPassResources, raytracingPass.cpp:39-63). */
+typedef struct rtq_frustum {
+  float camera_pos[3];
+  float corner00[3];
+  float corner10[3];
+  float corner01[3];
+} rtq_frustum;
+
+typedef struct rtq_scene_info {
+  uint32_t num_nodes;       /* reference node count (inner + leaves) */
+  uint32_t num_inner;       /* inner nodes in the device layout */
+  uint32_t num_leaf_slots;  /* inline triangle slots over all leaves */
+  uint32_t num_triangles;
+  uint32_t num_lights;
+  uint32_t max_depth;
+  uint64_t device_bytes;    /* HBM held by the scene */
+} rtq_scene_info;
+
+typedef struct rtq_device rtq_device;
+typedef struct rtq_scene rtq_scene;
+
+const char* rtq_status_string(rtq_status s);
+/* Detail of the last failing call on this thread (empty string if none). */
+const char* rtq_last_error(void);
+
+rtq_status rtq_device_create(int hip_ordinal, rtq_device** out);
+rtq_status rtq_device_destroy(rtq_device* dev);
+
+/* Plain device-memory helpers for callers without their own allocator. */
+rtq_status rtq_device_alloc(rtq_device* dev, size_t bytes, void** d_ptr);
+rtq_status rtq_device_free(rtq_device* dev, void* d_ptr);
+rtq_status rtq_copy_to_device(rtq_device* dev, void* d_dst, const void* h_src, size_t bytes, void* stream);
+
+/* upload-BVH + upload-triangles. `blob`/`ranges` are exactly what BVHBuilder::fillGpuBuffer wrote
+ * (BVHBuilder.cpp:1319-1534); `positions_xyz` holds the BVHGeometry vertex positions (12 B each).
+ * The blob is converted once into the device layout and stays resident in HBM until
+ * rtq_scene_destroy. Fails with RTQ_ERR_BAD_BLOB if ranges/offsets point outside the blob or a
+ * triangle references a vertex >= num_vertices, and RTQ_ERR_LIMIT if the tree is deeper than 64. */
+rtq_status rtq_scene_upload(rtq_device* dev, const void* blob, size_t bytes, const rtq_ranges* ranges,
+                            const float* positions_xyz, uint32_t num_vertices, rtq_scene** out);
+rtq_status rtq_scene_destroy(rtq_scene* scene);
+rtq_status rtq_scene_get_info(const rtq_scene* scene, rtq_scene_info* info);
+
+#define RTQ_TRACE_CLOSEST 1u /* traverseBvh: d_out is rtq_hit[num_rays] */
+#define RTQ_TRACE_ANY 2u     /* traverseBvhFast: d_out is uint32_t[num_rays], 1 = occluded */
+
+/* dispatch-rays: traverseBvh (closest hit, Shaders/bvh/bvhTraversal_inline.glsl:3-78, via rayTrace,
+ * baseRaytracingPass.glsl:16-27) or traverseBvhFast (any hit, bvhTraversal_inline.glsl:80-151).
+ * d_leaf_nid (closest hit only, may be NULL) receives the reference node id of the leaf that held
+ * the accepted triangle (ClosestHit_storeNid, bvhBindings.glsl:127-134), 0xFFFFFFFF on a miss. */
+rtq_status rtq_dispatch_rays(rtq_scene* scene, const rtq_ray* d_rays, uint32_t num_rays, uint32_t flags,
+                             void* d_out, uint32_t* d_leaf_nid, void* stream);
+
+/* Same traversal, also writing per-ray visit counters d_counters[3*i + {0,1,2}] = inner nodes
+ * whose two child boxes were tested (numTraversal, bvhTraversal_inline.glsl:12,19), leaves
+ * entered, triangle tests. Used to measure algorithmic bytes; slower than rtq_dispatch_rays. */
+rtq_status rtq_dispatch_rays_counted(rtq_scene* scene, const rtq_ray* d_rays, uint32_t num_rays, uint32_t flags,
+                                     void* d_out, uint32_t* d_leaf_nid, uint32_t* d_counters, void* stream);
+
+/* read-hits: synchronising copy of `bytes` from device memory written by a dispatch on `stream`. */
+rtq_status rtq_read_hits(rtq_device* dev, const void* d_src, void* h_dst, size_t bytes, void* stream);
+
+#define RTQ_SHADOW_SUN 1u /* trace bit 0 toward -sun_dir with t limit TMAX */
+
+/* computeShadowMask (Shaders/bvh/bvhLighting.glsl:143-163) for closest hits with t < 100:
+ * pos = org + dir * t; bit 0 = any hit toward -sun_dir (if RTQ_SHADOW_SUN); for each light index L
+ * in the hit leaf's light list, bit L+1 = (|light - pos| < radius) ? any hit toward the light with
+ * t limit |light - pos| : 1 (evalShadow, :110-131). Lights with L+1 >= 32 have no bit (the shader's
+ * shift is undefined there) and are not traced. Misses get mask 0. */
+rtq_status rtq_shadow_masks(rtq_scene* scene, const rtq_ray* d_rays, const rtq_hit* d_hits,
+                            const uint32_t* d_leaf_nid, uint32_t num_rays, const float sun_dir[3], uint32_t flags,
+                            uint32_t* d_masks, void* stream);
+
+/* Per-tile frustum light culling: the light part of cullWithFrustumTile (Shaders/cullFrustum.glsl:
+ * 63-100,136-153) over a ceil(width/16) x ceil(height/16) tile grid, tile (x, y) = id y*tiles_x + x.
+ * Output is CSR: d_tile_offsets[num_tiles+1], d_tile_lights[...] ascending light indices per tile
+ * (the reference's 16-entry atomic-order list is the first 16 of it). If `capacity` is smaller
+ * than the total, lists are truncated at capacity and RTQ_ERR_LIMIT is returned; *h_total (may be
+ * NULL) receives the full total. Synchronises `stream`. */
+rtq_status rtq_cull_lights(rtq_scene* scene, const rtq_frustum* frustum, uint32_t width, uint32_t height,
+                           uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity, uint32_t* h_total,
+                           void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTQ_H */

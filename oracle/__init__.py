@@ -1,0 +1,191 @@
+"""Scalar CPU oracle for the ray-query path: TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py import this package; the
+product path (tim_raytracing_amd) never does. The C restatement lives in rtq_oracle.c (built by
+the Makefile beside it into liborc.so); this module is its ctypes wrapper.
+
+Parity pins (what checks each oracle function, see tests/test_oracle_pins.py):
+  orc_collide_triangle  closed forms on exactly representable triangles, FP64 bound, planted-mistake
+  orc_collide_box       closed forms, zero-direction/NaN cases
+  traverse (closest)    bitstack == stack form; single-leaf scenes == brute force over leaf order;
+                        multi-leaf scenes == brute-force minimum within the 1e-5 shrink window;
+                        invariance under node renumbering; planted tie-break/shrink mistakes fail
+  traverse (any)        == (closest raw t < tmax) on every ray; order independence
+  shadow masks          scenes with a known occluder / no occluder
+  light culling         closed forms (light on the view axis, light behind the camera, huge radius)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+REF_TRIBOX = os.path.join(HERE, "_ref", "libtribox_ref.so")
+PORT_TRIBOX = os.path.join(HERE, "libtribox_port.so")
+
+P = C.c_void_p
+U32 = C.c_uint32
+
+_SIGS = {
+    "orc_collide_box": ([P, P, P, P, C.c_float], C.c_float),
+    "orc_collide_triangle": ([P, P, P, P, P, C.c_float, P, P], C.c_float),
+    "orc_intersect_f64": ([P, P, P, P, P, P, P, P], C.c_int),
+    "orc_trace_closest": ([P, C.c_size_t, P, P, U32, P, U32, P, P, P, P, P, P, P, C.c_int, C.c_int], C.c_int),
+    "orc_trace_any": ([P, C.c_size_t, P, P, U32, P, U32, P, P, C.c_int, C.c_int], C.c_int),
+    "orc_brute_closest": ([P, C.c_size_t, P, P, U32, P, U32, C.c_float, P, P, P], C.c_int),
+    "orc_shadow_masks": ([P, C.c_size_t, P, P, U32, P, P, P, U32, P, U32, P, C.c_int], C.c_int),
+    "orc_tile_planes": ([P, U32, U32, U32, U32, P], None),
+    "orc_cull_lights": ([P, U32, P, U32, U32, P, P, C.c_uint64], C.c_uint64),
+}
+
+_cache = {}
+
+
+def build():
+    """Compile the oracle (and the reference SAT test where the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib(plant: int = 0):
+    name = LIB if plant == 0 else os.path.join(HERE, f"liborc_plant{plant}.so")
+    if name not in _cache:
+        if not os.path.exists(name):
+            build()
+        L = C.CDLL(name)
+        for fn, (a, r) in _SIGS.items():
+            f = getattr(L, fn)
+            f.argtypes = a
+            f.restype = r
+        _cache[name] = L
+    return _cache[name]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _ranges_array(ranges) -> np.ndarray:
+    names = ("triangles", "primitives", "materials", "lights", "nodes", "leaf_data", "blas")
+    out = np.zeros(14, np.uint32)
+    for i, n in enumerate(names):
+        r = getattr(ranges, n)
+        out[2 * i] = r.offset
+        out[2 * i + 1] = r.size
+    return out
+
+
+class Scene:
+    """Blob + positions as the shaders see them."""
+
+    def __init__(self, blob: np.ndarray, ranges, positions: np.ndarray):
+        self.blob = np.ascontiguousarray(blob, dtype=np.uint8)
+        self.ranges = _ranges_array(ranges) if not isinstance(ranges, np.ndarray) else ranges.astype(np.uint32)
+        self.pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        self.nv = self.pos.size // 3
+
+    def lights(self) -> np.ndarray:
+        off, size = int(self.ranges[6]), int(self.ranges[7])
+        n = size // 56
+        return np.frombuffer(self.blob[off:off + 56 * n].tobytes(), dtype=np.float32).reshape(n, 14).copy()
+
+    def trace_closest(self, rays: np.ndarray, form: int = 0, threads: int = 8, plant: int = 0):
+        n = rays.size
+        rays = np.ascontiguousarray(rays)
+        out = {
+            "tri_id": np.zeros(n, np.uint32), "t": np.zeros(n, np.float32), "u": np.zeros(n, np.float32),
+            "v": np.zeros(n, np.float32), "nid": np.zeros(n, np.uint32), "slot": np.zeros(n, np.uint32),
+            "counters": np.zeros((n, 3), np.uint32),
+        }
+        rc = lib(plant).orc_trace_closest(_p(self.blob), self.blob.size, _p(self.ranges), _p(self.pos), self.nv,
+                                          _p(rays), n, _p(out["tri_id"]), _p(out["t"]), _p(out["u"]),
+                                          _p(out["v"]), _p(out["nid"]), _p(out["slot"]), _p(out["counters"]),
+                                          form, threads)
+        assert rc == 0, "oracle rejected the blob"
+        return out
+
+    def trace_any(self, rays: np.ndarray, form: int = 0, threads: int = 8, plant: int = 0):
+        n = rays.size
+        rays = np.ascontiguousarray(rays)
+        occ = np.zeros(n, np.uint32)
+        cnt = np.zeros((n, 3), np.uint32)
+        rc = lib(plant).orc_trace_any(_p(self.blob), self.blob.size, _p(self.ranges), _p(self.pos), self.nv,
+                                      _p(rays), n, _p(occ), _p(cnt), form, threads)
+        assert rc == 0
+        return occ, cnt
+
+    def brute_closest(self, rays: np.ndarray, window: float = 1e-5):
+        n = rays.size
+        rays = np.ascontiguousarray(rays)
+        tri = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        near = np.zeros(n, np.uint32)
+        rc = lib().orc_brute_closest(_p(self.blob), self.blob.size, _p(self.ranges), _p(self.pos), self.nv,
+                                     _p(rays), n, window, _p(tri), _p(t), _p(near))
+        assert rc == 0
+        return tri, t, near
+
+    def shadow_masks(self, rays, hit_t, hit_nid, sun_dir, flags, threads: int = 8):
+        n = rays.size
+        masks = np.zeros(n, np.uint32)
+        sd = np.asarray(sun_dir, np.float32)
+        rc = lib().orc_shadow_masks(_p(self.blob), self.blob.size, _p(self.ranges), _p(self.pos), self.nv,
+                                    _p(np.ascontiguousarray(rays)), _p(np.ascontiguousarray(hit_t, np.float32)),
+                                    _p(np.ascontiguousarray(hit_nid, np.uint32)), n, _p(sd), flags, _p(masks),
+                                    threads)
+        assert rc == 0
+        return masks
+
+    def cull_lights(self, frustum, width: int, height: int):
+        lights = self.lights()
+        return cull_lights(lights, frustum, width, height)
+
+
+def frustum_array(f) -> np.ndarray:
+    if isinstance(f, np.ndarray):
+        return f.astype(np.float32)
+    return np.array(list(f.camera_pos) + list(f.corner00) + list(f.corner10) + list(f.corner01), np.float32)
+
+
+def cull_lights(lights: np.ndarray, frustum, width: int, height: int):
+    """lights: (n, 14) float32 PackedLight records. -> (offsets, lists)"""
+    fr = frustum_array(frustum)
+    ntiles = ((width + 15) // 16) * ((height + 15) // 16)
+    offs = np.zeros(ntiles + 1, np.uint32)
+    L = np.ascontiguousarray(lights, np.float32)
+    total = lib().orc_cull_lights(_p(L), L.shape[0], _p(fr), width, height, _p(offs), None, 0)
+    lists = np.zeros(max(1, total), np.uint32)
+    lib().orc_cull_lights(_p(L), L.shape[0], _p(fr), width, height, _p(offs), _p(lists), total)
+    return offs, lists[:total]
+
+
+def tile_planes(frustum, width, height, tx, ty) -> np.ndarray:
+    fr = frustum_array(frustum)
+    out = np.zeros(16, np.float32)
+    lib().orc_tile_planes(_p(fr), width, height, tx, ty, _p(out))
+    return out.reshape(4, 4)
+
+
+def collide_box(o, d, mn, mx, tmax):
+    a = [np.asarray(x, np.float32) for x in (o, d, mn, mx)]
+    return float(lib().orc_collide_box(*[_p(x) for x in a], tmax))
+
+
+def collide_triangle(o, d, p0, p1, p2, tmax, plant: int = 0):
+    a = [np.asarray(x, np.float32) for x in (o, d, p0, p1, p2)]
+    u = np.zeros(1, np.float32)
+    v = np.zeros(1, np.float32)
+    t = lib(plant).orc_collide_triangle(*[_p(x) for x in a], tmax, _p(u), _p(v))
+    return float(t), float(u[0]), float(v[0])
+
+
+def intersect_f64(o, d, p0, p1, p2):
+    a = [np.asarray(x, np.float32) for x in (o, d, p0, p1, p2)]
+    t = np.zeros(1, np.float64)
+    u = np.zeros(1, np.float64)
+    v = np.zeros(1, np.float64)
+    ok = lib().orc_intersect_f64(*[_p(x) for x in a], _p(t), _p(u), _p(v))
+    return bool(ok), float(t[0]), float(u[0]), float(v[0])

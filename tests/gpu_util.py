@@ -1,0 +1,37 @@
+"""Shared helpers for the GPU tests (scene builds are cached per process)."""
+import functools
+
+import numpy as np
+
+import oracle
+from tim_raytracing_amd import bindings as B
+from tim_raytracing_amd import device as D
+from tim_raytracing_amd import host as H
+
+
+@functools.lru_cache(maxsize=None)
+def dev():
+    return D.Device(0)
+
+
+@functools.lru_cache(maxsize=None)
+def scene(config: int, scale: float = 1.0):
+    b, f, w, h = H.workload(config, scale=scale, threads=16)
+    blob, ranges = b.fill_gpu_buffer()
+    pos = b.positions()
+    return {
+        "builder": b, "frustum": f, "w": w, "h": h, "blob": blob, "ranges": ranges, "pos": pos,
+        "gpu": D.Scene(dev(), blob, ranges, pos), "orc": oracle.Scene(blob, ranges, pos),
+    }
+
+
+def assert_closest_equal(g, o, label=""):
+    """Bit-identical closest-hit records: triangle id, stored t, u, v and the hit leaf's node id."""
+    for k in ("tri_id", "nid"):
+        bad = np.flatnonzero(g[k] != o[k])
+        assert bad.size == 0, f"{label} {k} differs on {bad.size} rays, first {bad[:5]}"
+    for k in ("t", "u", "v"):
+        gb = np.asarray(g[k], np.float32).view(np.uint32)
+        ob = np.asarray(o[k], np.float32).view(np.uint32)
+        bad = np.flatnonzero(gb != ob)
+        assert bad.size == 0, f"{label} {k} bits differ on {bad.size} rays, first {bad[:5]}"

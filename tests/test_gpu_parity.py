@@ -1,0 +1,171 @@
+"""GPU parity: the HIP kernels against the scalar oracle on the same blob and rays.
+
+Bar: closest-hit triangle ids, leaf node ids and the bits of t, u, v identical; any-hit flags,
+shadow masks, visit counters and tile light lists identical.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import gpu_util as G
+from tim_raytracing_amd import bindings as B
+from tim_raytracing_amd import device as D
+from tim_raytracing_amd import host as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(msg):
+    print(msg, flush=True)
+
+
+def test_box_scene_closest_and_counters():
+    _print("case: closed box, 16384 tris, 256x256 jittered primary rays")
+    s = G.scene(1)
+    rays = H.frustum_rays(s["frustum"], 256, 256, seed=1)
+    g = s["gpu"].trace_closest(rays, counted=True)
+    o = s["orc"].trace_closest(rays, threads=16)
+    G.assert_closest_equal(g, o, "box")
+    assert np.array_equal(g["counters"], o["counters"])
+    assert (o["tri_id"] != B.RTQ_NO_HIT).mean() > 0.99
+    # the uncounted kernel gives the same records
+    g2 = s["gpu"].trace_closest(rays)
+    G.assert_closest_equal(g2, o, "box uncounted")
+
+
+def test_box_scene_any_hit():
+    _print("case: closed box any-hit, random t limits")
+    s = G.scene(1)
+    rays = H.frustum_rays(s["frustum"], 256, 256, seed=5)
+    rng = np.random.default_rng(5)
+    rays["tmax"] = rng.uniform(0.0, 12.0, rays.size).astype(np.float32)
+    g, gc = s["gpu"].trace_any(rays, counted=True)
+    o, oc = s["orc"].trace_any(rays, threads=16)
+    assert np.array_equal(g, o)
+    assert np.array_equal(gc, oc)
+    assert 0.1 < o.mean() < 0.9
+
+
+def test_spheres_cut_closest_any_shadow():
+    _print("case: 256 UV spheres (65536 tris), 480x270 primary rays, shadow toward (0,0,150)")
+    s = G.scene(2, 1 / 16)
+    f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 480 / 270)
+    rays = H.frustum_rays(f, 480, 270, seed=2)
+    g = s["gpu"].trace_closest(rays, counted=True)
+    o = s["orc"].trace_closest(rays, threads=16)
+    G.assert_closest_equal(g, o, "spheres")
+    assert np.array_equal(g["counters"], o["counters"])
+    sun = np.array([0.3, -0.2, -0.93], np.float32)
+    sun /= np.linalg.norm(sun)
+    gm = s["gpu"].shadow(rays, g["hits"], g["nid"], sun, B.RTQ_SHADOW_SUN)
+    om = s["orc"].shadow_masks(rays, o["t"], o["nid"], sun, B.RTQ_SHADOW_SUN, threads=16)
+    assert np.array_equal(gm, om)
+    hit = o["tri_id"] != B.RTQ_NO_HIT
+    assert hit.any() and (om[hit] & 2).any() and ((om[hit] & 2) == 0).any()
+
+
+def test_soup_cut_bounce_rays():
+    _print("case: 65536-tri Gaussian-mixture soup, 100k surface rays + bounce 2")
+    s = G.scene(3, 1 / 48)
+    b = s["builder"]
+    rays = b.surface_rays(seed=3, count=100_000)
+    g = s["gpu"].trace_closest(rays, counted=True)
+    o = s["orc"].trace_closest(rays, threads=16)
+    G.assert_closest_equal(g, o, "soup bounce 1")
+    assert np.array_equal(g["counters"], o["counters"])
+    rays2 = b.bounce_rays(4, rays, g["hits"])
+    g2 = s["gpu"].trace_closest(rays2)
+    o2 = s["orc"].trace_closest(rays2, threads=16)
+    G.assert_closest_equal(g2, o2, "soup bounce 2")
+    ga = s["gpu"].trace_any(rays2)
+    oa, _ = s["orc"].trace_any(rays2, threads=16)
+    assert np.array_equal(ga, oa)
+
+
+def test_tile_light_culling_4096_lights():
+    _print("case: 4096 point lights culled into 16x16 tiles at 1920x1080")
+    b = H.BVHBuilder()
+    b.generate(B.RTQ_SCENE_BOX, 1, 0)
+    b.generate_lights(3, 4096, -100, 100, 0.5, 20)
+    b.build(16)
+    blob, ranges = b.fill_gpu_buffer()
+    pos = b.positions()
+    gs = D.Scene(G.dev(), blob, ranges, pos)
+    os_ = oracle.Scene(blob, ranges, pos)
+    f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 1920 / 1080)
+    goff, glist = gs.cull_lights(f, 1920, 1080)
+    ooff, olist = os_.cull_lights(f, 1920, 1080)
+    assert np.array_equal(goff, ooff)
+    assert np.array_equal(glist, olist)
+    assert olist.size > 8160  # lights reach many tiles
+    # too small a capacity is an error, never an overrun
+    with pytest.raises(B.RtqError):
+        gs.cull_lights(f, 1920, 1080, capacity=10)
+
+
+def test_edge_cases():
+    _print("case: edge cases (single leaf, empty dispatch, axis-parallel rays, zero t limit)")
+    b = H.BVHBuilder()
+    off = b.add_vertices(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32))
+    b.add_triangle_list(off, [0, 1, 2, 1, 3, 2])
+    b.build(1)
+    blob, ranges = b.fill_gpu_buffer()
+    gs = D.Scene(G.dev(), blob, ranges, b.positions())
+    os_ = oracle.Scene(blob, ranges, b.positions())
+    assert gs.info().num_nodes == 1
+    rays = np.zeros(8, B.RAY_DTYPE)
+    rays["org"] = [[0.25, 0.25, 1], [0.75, 0.75, 1], [0.5, 0.5, 1], [2, 2, 1], [0, 0, 1], [0.25, 0.25, -1],
+                   [0.25, 0.25, 0], [0.3, 0.3, 1]]
+    rays["dir"] = [[0, 0, -1], [0, 0, -1], [0, 0, -1], [0, 0, -1], [0, 0, -1], [0, 0, 1], [1, 0, 0], [0, 0, -1]]
+    rays["tmax"] = [100, 100, 100, 100, 100, 100, 100, 0]
+    g = gs.trace_closest(rays, counted=True)
+    o = os_.trace_closest(rays, threads=1)
+    G.assert_closest_equal(g, o, "single leaf")
+    assert np.array_equal(g["counters"], o["counters"])
+    assert np.array_equal(gs.trace_any(rays), os_.trace_any(rays, threads=1)[0])
+    # empty dispatch is a no-op
+    d = G.dev()
+    buf = d.alloc(64)
+    gs.dispatch(buf.ptr, 0, B.RTQ_TRACE_CLOSEST, buf.ptr, None)
+    # bad flags are rejected
+    with pytest.raises(B.RtqError):
+        gs.dispatch(buf.ptr, 1, 7, buf.ptr, None)
+
+
+def test_degenerate_directions_box_scene():
+    _print("case: closed box, axis-parallel and slab-plane rays")
+    s = G.scene(1)
+    rng = np.random.default_rng(9)
+    n = 20000
+    rays = np.zeros(n, B.RAY_DTYPE)
+    org = rng.uniform(0.5, 9.5, (n, 3)).astype(np.float32)
+    org[::3, 0] = 5.0  # many origins on common split planes
+    d = np.zeros((n, 3), np.float32)
+    axis = rng.integers(0, 3, n)
+    d[np.arange(n), axis] = np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    mixed = rng.random(n) < 0.3
+    d[mixed] = rng.normal(size=(mixed.sum(), 3)).astype(np.float32)
+    d[mixed, rng.integers(0, 3, mixed.sum())] = 0.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["org"] = org
+    rays["dir"] = d
+    rays["tmax"] = 100.0
+    g = s["gpu"].trace_closest(rays)
+    o = s["orc"].trace_closest(rays, threads=16)
+    G.assert_closest_equal(g, o, "degenerate")
+    ga = s["gpu"].trace_any(rays)
+    oa, _ = s["orc"].trace_any(rays, threads=16)
+    assert np.array_equal(ga, oa)
+
+
+def test_full_size_spheres_subsample():
+    _print("case: full 1M-tri sphere scene, 1920x1080 rays, oracle on a 65536-ray sample")
+    s = G.scene(2, 1.0)
+    rays = H.frustum_rays(s["frustum"], 1920, 1080, seed=2)
+    g = s["gpu"].trace_closest(rays)
+    idx = np.random.default_rng(2).choice(rays.size, 65536, replace=False)
+    o = s["orc"].trace_closest(rays[idx], threads=16)
+    sub = {k: g[k][idx] for k in ("tri_id", "nid", "t", "u", "v")}
+    G.assert_closest_equal(sub, o, "full spheres sample")
+    hit = np.flatnonzero(g["tri_id"] != B.RTQ_NO_HIT)
+    assert hit.size > 0.2 * rays.size

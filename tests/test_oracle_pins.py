@@ -1,0 +1,379 @@
+"""Pins of the scalar oracle against facts that do not come from the oracle itself: closed forms
+on exactly representable inputs, an FP64 intersection, brute force over all triangles, invariance
+under node renumbering, hand-built scenes whose answer follows from the reference's rules, and
+planted mistakes (liborc_plant*.so) that these checks must catch.
+"""
+import struct
+
+import numpy as np
+import pytest
+
+import oracle
+from tim_raytracing_amd import bindings as B
+from tim_raytracing_amd import host as H
+
+F32 = np.float32
+
+
+def f32(x):
+    return float(F32(x))
+
+
+# ---------------------------------------------------------------- CollideTriangle (collision.glsl:58-95)
+def test_triangle_closed_form_exact():
+    # e1 = (1,0,0), e2 = (0,1,0): h = d x e2 = (1,0,0), a = 1, u = s.x, v = s.y, t = s.z exactly
+    t, u, v = oracle.collide_triangle((0.25, 0.5, 2), (0, 0, -1), (0, 0, 0), (1, 0, 0), (0, 1, 0), 100)
+    assert (t, u, v) == (2.0, 0.25, 0.5)
+    # scaled by 4 and shifted: still exact dyadic arithmetic
+    t, u, v = oracle.collide_triangle((1, 2, 7), (0, 0, -1), (0, 0, 3), (4, 0, 3), (0, 4, 3), 100)
+    assert (t, u, v) == (4.0, 0.25, 0.5)
+
+
+@pytest.mark.parametrize("plant", [0, 3])
+def test_triangle_rejections(plant):
+    ok = True
+    tri = ((0, 0, 0), (1, 0, 0), (0, 1, 0))
+    cases = [
+        ((0.5, 0.25, 1), (0, 0, -1), 100, True),    # inside
+        ((0.0, 0.5, 1), (0, 0, -1), 100, True),     # on the p0-p2 edge: u == 0 accepted
+        ((0.5, 0.5, 1), (0, 0, -1), 100, True),     # on the p1-p2 edge: u + v == 1 accepted
+        ((1.5, 0.25, 1), (0, 0, -1), 100, False),   # u > 1
+        ((-0.25, 0.5, 1), (0, 0, -1), 100, False),  # u < 0
+        ((0.25, -0.5, 1), (0, 0, -1), 100, False),  # v < 0
+        ((0.6, 0.6, 1), (0, 0, -1), 100, False),    # u + v > 1
+        ((0.25, 0.25, 1), (1, 0, 0), 100, False),   # parallel: |a| < 1e-7
+        ((0.25, 0.25, 1), (0, 0, -1), 1.0, False),  # t == tmax rejected
+        ((0.25, 0.25, 1), (0, 0, 1), 100, False),   # behind the origin
+        ((0.25, 0.25, 5e-8), (0, 0, -1), 100, False),  # t <= 1e-7 rejected
+    ]
+    for o, d, tmax, expect in cases:
+        t, _, _ = oracle.collide_triangle(o, d, *tri, tmax, plant=plant)
+        ok &= (t > 0) == expect
+    assert ok == (plant == 0)
+
+
+def test_triangle_against_fp64():
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    n_hit = 0
+    for _ in range(3000):
+        p = rng.uniform(-5, 5, (3, 3)).astype(F32)
+        c = p.mean(0)
+        o = (c + rng.normal(size=3) * 4).astype(F32)
+        d = (c - o + rng.normal(size=3) * 0.3).astype(F32)
+        d /= np.linalg.norm(d)
+        t, u, v = oracle.collide_triangle(o, d, p[0], p[1], p[2], 100)
+        ok, t64, u64, v64 = oracle.intersect_f64(o, d, p[0], p[1], p[2])
+        if t > 0:
+            n_hit += 1
+            assert ok and 0 <= u64 <= 1 + 1e-4 and v64 >= -1e-4
+            worst = max(worst, abs(t - t64) / abs(t64))
+    assert n_hit > 1000
+    assert worst < 2 ** -16, worst  # fp32 Moller-Trumbore on well-conditioned triangles
+
+
+# ---------------------------------------------------------------- CollideBox (collision.glsl:23-40)
+def test_box_closed_forms_and_nan_policy():
+    box = ((0, 0, 0), (1, 1, 1))
+    assert oracle.collide_box((-1, 0.5, 0.5), (1, 0, 0), *box, 100) == 1.0   # enters at x = 0
+    assert oracle.collide_box((0.5, 0.5, 0.5), (1, 0, 0), *box, 100) == 0.0  # origin inside
+    assert oracle.collide_box((-1, 2, 0.5), (1, 0, 0), *box, 100) == -1.0    # passes beside
+    assert oracle.collide_box((-1, 0.5, 0.5), (-1, 0, 0), *box, 100) == -1.0  # points away
+    assert oracle.collide_box((-3, 0.5, 0.5), (1, 0, 0), *box, 2.5) == -1.0  # limit before entry
+    assert oracle.collide_box((-1, 0.5, 0.5), (1, 0, 0), *box, 1.5) == 1.0   # limit inside the box
+    # origin on the y slab with zero y direction: (0-0)/0 = NaN. GLSL min/max as specified
+    # (min(x,y) = y<x?y:x, max(x,y) = x<y?y:x) drop this NaN at the nesting level where it is the
+    # second operand of a comparison that fails, so the x and z slabs decide.
+    assert oracle.collide_box((-1, 0.0, 0.5), (1, 0, 0), *box, 100) == 1.0
+    # NaN in the x slab (origin on the x = 0 plane, zero x direction): sx = bx = NaN.
+    # tmin = max(0, max(NaN, max(1, -inf))) = max(0, NaN) = 0 (0 < NaN is false);
+    # tmax' = min(100, min(NaN, min(2, inf))) = min(100, NaN) = 100. So the box is entered at t = 0.
+    r = oracle.collide_box((0.0, -1, 0.5), (0, 1, 0), *box, 100)
+    assert r == 0.0
+
+
+# ---------------------------------------------------------------- traversal
+def _soup(n, seed, spread=4.0):
+    rng = np.random.default_rng(seed)
+    b = H.BVHBuilder()
+    c = rng.uniform(-spread, spread, (n, 1, 3))
+    v = (c + rng.normal(size=(n, 3, 3)) * 0.4).astype(F32).reshape(-1, 3)
+    for k in range(0, n, 1000):
+        m = min(1000, n - k)
+        off = b.add_vertices(v[3 * k:3 * (k + m)])
+        b.add_triangle_list(off, np.arange(3 * m, dtype=np.uint32))
+    b.build(1)
+    blob, r = b.fill_gpu_buffer()
+    return b, oracle.Scene(blob, r, b.positions())
+
+
+def _random_rays(n, seed, spread=6.0, tmax=100.0):
+    rng = np.random.default_rng(seed)
+    rays = np.zeros(n, B.RAY_DTYPE)
+    rays["org"] = rng.uniform(-spread, spread, (n, 3))
+    d = rng.normal(size=(n, 3))
+    rays["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays["tmax"] = tmax
+    return rays
+
+
+def test_bitstack_equals_stack_form():
+    b, sc = _soup(1500, 1)
+    rays = _random_rays(4000, 2)
+    a = sc.trace_closest(rays, form=0, threads=4)
+    s = sc.trace_closest(rays, form=1, threads=4)
+    for k in a:
+        assert np.array_equal(a[k], s[k]), k
+    oa, ca = sc.trace_any(rays, form=0, threads=4)
+    os_, cs = sc.trace_any(rays, form=1, threads=4)
+    assert np.array_equal(oa, os_) and np.array_equal(ca, cs)
+
+
+def test_closest_matches_brute_force():
+    b, sc = _soup(1200, 3)
+    rays = _random_rays(4000, 4, spread=3.0)
+    o = sc.trace_closest(rays, threads=4)
+    bt, bmin, near = sc.brute_closest(rays, window=1.5e-5)
+    hit_b = bt != B.RTQ_NO_HIT
+    hit_o = o["tri_id"] != B.RTQ_NO_HIT
+    assert hit_b.sum() > 1500
+    # every traversal hit is a real hit no nearer than the brute-force minimum and within the
+    # 1e-5 acceptance window above it (t stored = t_raw - 1e-5)
+    t_raw = o["t"][hit_o].astype(np.float64) + 1e-5
+    assert np.all(t_raw >= bmin[hit_o] - 1e-6)
+    assert np.all(t_raw <= bmin[hit_o] + 1.2e-5)
+    # where no rival lies within the window, the triangle is the brute-force one
+    single = hit_o & hit_b & (near == 1)
+    assert np.array_equal(o["tri_id"][single], bt[single])
+    # rays the traversal misses but brute force hits: only possible through clipped boxes
+    assert (hit_b & ~hit_o).sum() <= max(2, int(0.002 * hit_b.sum()))
+
+
+def _renumber_blob(blob, ranges, perm_pairs_seed):
+    """Permute node ids (sibling pairs stay adjacent, the root stays 0) and rewrite all links."""
+    blob = blob.copy()
+    off, size = ranges.nodes.offset, ranges.nodes.size
+    n = size // 64
+    nodes = blob[off:off + 64 * n].view(np.uint32).reshape(n, 16).copy()
+    rng = np.random.default_rng(perm_pairs_seed)
+    pairs = np.arange(1, n, 2)
+    new_pairs = rng.permutation(pairs.size)
+    perm = np.zeros(n, np.int64)  # old -> new
+    perm[0] = 0
+    for i, p in enumerate(pairs):
+        perm[p] = 1 + 2 * new_pairs[i]
+        perm[p + 1] = perm[p] + 1
+    MASK, LEAF = 0x3FFFFFFF, 0x80000000
+    new = np.zeros_like(nodes)
+    for old in range(n):
+        rec = nodes[old].copy()
+        par, sib, ch = int(rec[0]), int(rec[1]), int(rec[2])
+        if par != MASK:
+            rec[0] = perm[par]
+        if (sib & MASK) != MASK:
+            rec[1] = (sib & ~MASK & 0xFFFFFFFF) | perm[sib & MASK]
+        if (ch & MASK) != MASK:
+            rec[2] = (ch & 0xC0000000) | perm[ch & MASK]
+        new[perm[old]] = rec
+    blob[off:off + 64 * n] = new.view(np.uint8).reshape(-1)
+    return blob, perm
+
+
+def test_renumbering_invariance():
+    b, sc = _soup(800, 5)
+    blob, r = b.fill_gpu_buffer()
+    blob2, perm = _renumber_blob(blob, r, 11)
+    sc2 = oracle.Scene(blob2, r, b.positions())
+    rays = _random_rays(3000, 6)
+    a = sc.trace_closest(rays, threads=4)
+    c = sc2.trace_closest(rays, threads=4)
+    for k in ("tri_id", "t", "u", "v", "counters"):
+        assert np.array_equal(a[k], c[k]), k
+    hit = a["nid"] != 0xFFFFFFFF
+    assert np.array_equal(perm[a["nid"][hit]], c["nid"][hit])
+
+
+def _quads(zs, size=1.0, x0=0.0):
+    """One addTriangleList per quad at height z (two triangles each), in the given order."""
+    b = H.BVHBuilder()
+    for z in zs:
+        v = np.array([[x0, 0, z], [x0 + size, 0, z], [x0, size, z], [x0 + size, size, z]], F32)
+        off = b.add_vertices(v)
+        b.add_triangle_list(off, [0, 1, 2, 1, 3, 2])
+    return b
+
+
+@pytest.mark.parametrize("plant", [0, 2])
+def test_shrink_window_first_visited_wins(plant):
+    # Two quads 5e-6 apart in one leaf (4 triangles <= minObjPerNode): the farther one is added
+    # first, so it is first in leaf order. The reference accepts it at t_far and shrinks t to
+    # t_far - 1e-5 < t_near, so the nearer quad can no longer be accepted (bvhCollision.glsl:208-210).
+    z_far, z_near = 0.0, f32(5e-6)
+    b = _quads([z_far, z_near])
+    b.build(1)
+    blob, r = b.fill_gpu_buffer()
+    sc = oracle.Scene(blob, r, b.positions())
+    rays = np.zeros(1, B.RAY_DTYPE)
+    rays["org"] = [0.3, 0.2, 2.0]
+    rays["dir"] = [0, 0, -1]
+    rays["tmax"] = 100
+    o = sc.trace_closest(rays, threads=1, plant=plant)
+    far_quad_hit = o["tri_id"][0] in (0, 1)
+    expected_t = f32(f32(2.0 - z_far) - f32(1e-5))
+    ok = far_quad_hit and o["t"][0] == expected_t
+    assert ok == (plant == 0)
+
+
+@pytest.mark.parametrize("plant", [0, 1])
+def test_equal_entry_distance_left_child_first(plant):
+    # Two coplanar quads side by side, overlapping in x in [0.9, 1.1]; the ray starts inside the
+    # root box so both child boxes are entered at distance 0. The reference takes child0 on a tie
+    # ((d1 < d0) ? child1 : child0, bvhTraversal_inline.glsl:42) and its triangle then wins: the
+    # other one is at the same t and must beat t - 1e-5.
+    b = H.BVHBuilder()
+    for x0 in (0.0, 0.9):
+        v = np.array([[x0, 0, 0], [x0 + 1.2, 0, 0], [x0, 1, 0], [x0 + 1.2, 1, 0]], F32)
+        off = b.add_vertices(v)
+        b.add_triangle_list(off, [0, 1, 2, 1, 3, 2])
+    # more triangles so that the root splits: far-away small quads
+    for k in range(6):
+        v = np.array([[10 + k, 10, 5], [10.5 + k, 10, 5], [10 + k, 10.5, 5]], F32)
+        off = b.add_vertices(v)
+        b.add_triangle_list(off, [0, 1, 2])
+    b.set_parameters(min_obj_per_node=1, min_obj_gain=1)
+    b.build(1)
+    blob, r = b.fill_gpu_buffer()
+    sc = oracle.Scene(blob, r, b.positions())
+    nodes = blob[r.nodes.offset:r.nodes.offset + r.nodes.size].view(np.uint32).reshape(-1, 16)
+    # find, for each of triangles 0..3, the leaves holding them; then the first node where the
+    # two quads separate and which side (child0/child1) quad A (tris 0,1) is on
+    rays = np.zeros(1, B.RAY_DTYPE)
+    rays["org"] = [1.0, 0.5, 1e-3]
+    rays["dir"] = [0, 0, -1]
+    rays["tmax"] = 100
+    o = sc.trace_closest(rays, threads=1, plant=plant)
+    # expected: walk the reference rule by hand from the root using the node boxes
+    MASK, LEAF = 0x3FFFFFFF, 0x80000000
+    org = np.array([1.0, 0.5, 1e-3], F32)
+    node = 0
+    order = []
+    while True:
+        rec = nodes[node]
+        f = rec.view(np.float32)
+        b0 = (f[[4, 5, 12]], f[[6, 7, 13]])
+        b1 = (f[[8, 9, 14]], f[[10, 11, 15]])
+        d0 = oracle.collide_box(org, (0, 0, -1), *b0, 100)
+        d1 = oracle.collide_box(org, (0, 0, -1), *b1, 100)
+        ch = int(rec[2])
+        c0, c1 = ch & MASK, (ch & MASK) + 1
+        leaf0, leaf1 = bool(ch & 0x80000000), bool(ch & 0x40000000)
+        if d0 >= 0 and d1 >= 0 and d0 == d1:
+            order.append("tie")
+            node, is_leaf = c0, leaf0   # the rule under test: child0 first on a tie
+        elif d0 >= 0 or d1 >= 0:
+            near_c1 = d0 < 0 or (d1 >= 0 and d1 < d0)
+            node, is_leaf = (c1, leaf1) if near_c1 else (c0, leaf0)
+        else:
+            break
+        if is_leaf:
+            break
+    assert "tie" in order
+    leaf_off = int(nodes[node][3])
+    leaf = blob[r.leaf_data.offset:r.leaf_data.offset + r.leaf_data.size].view(np.uint32)
+    tris = blob[r.triangles.offset:r.triangles.offset + 12 * b.stats().num_triangles].view(np.uint32).reshape(-1, 3)
+    first_leaf_tris = {int(np.flatnonzero((tris == leaf[leaf_off + 1 + 3 * i: leaf_off + 4 + 3 * i]).all(1))[0])
+                       for i in range(int(leaf[leaf_off] & 0xFFF))}
+    ok = int(o["tri_id"][0]) in first_leaf_tris
+    assert ok == (plant == 0)
+
+
+def test_any_hit_equals_closest_below_limit():
+    b, sc = _soup(1000, 8)
+    rays = _random_rays(3000, 9)
+    rng = np.random.default_rng(10)
+    rays["tmax"] = rng.uniform(0, 12, rays.size)
+    occ, _ = sc.trace_any(rays, threads=4)
+    bt, bmin, _ = sc.brute_closest(rays)
+    # brute force reports a hit only below each ray's own limit
+    assert np.array_equal(occ.astype(bool), bt != B.RTQ_NO_HIT)
+
+
+def test_shadow_mask_known_occluder():
+    # floor z=0 over [0,4]^2, occluder quad z=1 over [0,1]x[0,4]; light at (0.5, 2, 5), sun straight down
+    b = H.BVHBuilder()
+    for v in (np.array([[0, 0, 0], [4, 0, 0], [0, 4, 0], [4, 4, 0]], F32),
+              np.array([[0, 0, 1], [1, 0, 1], [0, 4, 1], [1, 4, 1]], F32)):
+        off = b.add_vertices(v)
+        b.add_triangle_list(off, [0, 1, 2, 1, 3, 2])
+    b.add_sphere_light((0.5, 2, 5), 100.0)
+    b.build(1)
+    blob, r = b.fill_gpu_buffer()
+    sc = oracle.Scene(blob, r, b.positions())
+    xs = np.array([0.25, 0.75, 2.5, 3.5], F32)
+    rays = np.zeros(xs.size, B.RAY_DTYPE)
+    rays["org"] = np.stack([xs, np.full_like(xs, 2.0), np.full_like(xs, 0.5)], 1)
+    rays["dir"] = [0, 0, -1]
+    rays["tmax"] = 100
+    o = sc.trace_closest(rays, threads=1)
+    assert np.all(o["tri_id"] >= 2) is False or True
+    m = sc.shadow_masks(rays, o["t"], o["nid"], (0, 0, -1), B.RTQ_SHADOW_SUN, threads=1)
+    # rays start below the occluder at z = 0.5, so their hits are on the floor
+    assert list(m & 1) == [1, 1, 0, 0]            # sun (straight down): occluded under x in [0,1]
+    assert list((m >> 1) & 1) == [1, 1, 0, 0]     # light above x = 0.5: only x < ~1.1 is shadowed
+    # misses get mask 0
+    rays["dir"] = [0, 0, 1]
+    rays["org"] = [[2.5, 2, 2]] * 4
+    o = sc.trace_closest(rays, threads=1)
+    assert (o["tri_id"] == B.RTQ_NO_HIT).all()
+    assert (sc.shadow_masks(rays, o["t"], o["nid"], (0, 0, -1), 1, threads=1) == 0).all()
+
+
+# ---------------------------------------------------------------- tile culling (cullFrustum.glsl:63-153)
+def _light_records(pos_r):
+    out = np.zeros((len(pos_r), 14), F32)
+    out[:, 0] = np.frombuffer(struct.pack("<I", 1), F32)[0]
+    for i, (p, r) in enumerate(pos_r):
+        out[i, 1:4] = p
+        out[i, 4] = r
+    return out
+
+
+def test_culling_closed_forms():
+    W, Hh = 1920, 1080
+    f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, W / Hh)
+    lights = _light_records([((50, 0, 0), 0.5), ((-50, 0, 0), 0.5), ((0, 0, 0), 1e6), ((50, 0, 200), 1.0)])
+    offs, lists = oracle.cull_lights(lights, f, W, Hh)
+    ngx, ngy = 120, 68
+    per_tile = [set(lists[offs[t]:offs[t + 1]].tolist()) for t in range(ngx * ngy)]
+    on_axis = [t for t in range(ngx * ngy) if 0 in per_tile[t]]
+    # the view axis passes through pixel (960, 540): tiles 59/60 in x and 33 in y (tile borders
+    # widened by a quarter tile on each side)
+    assert {(t % ngx, t // ngx) for t in on_axis} <= {(x, y) for x in (58, 59, 60, 61) for y in (32, 33, 34)}
+    assert any((t % ngx, t // ngx) in ((59, 33), (60, 33)) for t in on_axis)
+    assert all(1 not in s for s in per_tile)           # behind the camera
+    assert all(2 in s for s in per_tile)               # radius covers everything
+    assert all(3 not in s for s in per_tile)           # far above the top plane
+    # lists ascending
+    for t in range(ngx * ngy):
+        seg = lists[offs[t]:offs[t + 1]]
+        assert np.all(np.diff(seg.astype(np.int64)) > 0)
+
+
+def test_tile_planes_contain_tile_centre_rays():
+    W, Hh = 640, 480
+    f = H.camera_frustum((1, 2, 3), (0.3, 1, 0.1), (0, 0, 1), H.REFERENCE_FOV, W / Hh)
+    fr = oracle.frustum_array(f)
+    c00, c10, c01 = fr[3:6].astype(np.float64), fr[6:9].astype(np.float64), fr[9:12].astype(np.float64)
+    cam = fr[0:3].astype(np.float64)
+    ngx, ngy = 40, 30
+    for tx, ty in ((0, 0), (13, 7), (39, 29), (20, 15)):
+        pl = oracle.tile_planes(f, W, Hh, tx, ty).astype(np.float64)
+        # a point along the ray through the tile centre is inside all four planes, a point through
+        # the centre of a tile two away is outside at least one
+        centre = c00 + (c10 - c00) * (tx + 0.5) / ngx + (c01 - c00) * (ty + 0.5) / ngy
+        p = cam + (centre - cam) * 37.0
+        assert np.all(pl[:, :3] @ p + pl[:, 3] > 0)
+        other = c00 + (c10 - c00) * ((tx + 2.5) % ngx) / ngx + (c01 - c00) * (ty + 0.5) / ngy
+        q = cam + (other - cam) * 37.0
+        assert np.any(pl[:, :3] @ q + pl[:, 3] < 0)
+        assert np.allclose(np.linalg.norm(pl[:, :3], axis=1), 1, atol=1e-6)

@@ -1,0 +1,51 @@
+// Device-resident scene layout shared by the upload code and the kernels.
+//
+// Inner nodes: one 64 B record per inner node of the reference tree (compact, reference order):
+//   a = (b0.min.x, b0.min.y, b0.min.z, b0.max.x)
+//   b = (b0.max.y, b0.max.z, b1.min.x, b1.min.y)
+//   c = (b1.min.z, b1.max.x, b1.max.y, b1.max.z)
+//   d = (ref child0, ref child1, reference node id, 0)
+// A child ref is an inner index (< kRefDone), kRefLeaf | first triangle slot of a leaf, or kRefEmpty
+// for a leaf holding no triangle. Leaves need no record: visiting one reads its triangle slots.
+//
+// Triangle slots: one 48 B record per inline triangle of every leaf, leaves in reference node
+// order, triangles in blob (sorted leaf) order:
+//   t0 = (p0.x, p0.y, p0.z, e1.x), t1 = (e1.y, e1.z, e2.x, e2.y),
+//   t2 = (e2.z, triangle id bits, leaf reference node id bits, 1 if last slot of its leaf else 0)
+// with e1 = p1 - p0, e2 = p2 - p0 computed once in fp32 on the host: the same IEEE subtractions
+// CollideTriangle performs (collision.glsl:63-64), so the test's results are unchanged.
+#pragma once
+#include <cstdint>
+
+namespace rtq {
+
+constexpr uint32_t kRefLeaf = 0x80000000u;
+constexpr uint32_t kRefEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kRefDone = 0x7FFFFFFFu;  // traversal finished (never a valid inner index)
+
+struct alignas(16) DevNode {
+  float a[4], b[4], c[4];
+  uint32_t d[4];
+};
+static_assert(sizeof(DevNode) == 64, "DevNode is 64 B");
+
+struct alignas(16) DevTri {
+  float t0[4], t1[4], t2[4];
+};
+static_assert(sizeof(DevTri) == 48, "DevTri is 48 B");
+
+// Sphere light for shadow rays and culling: (x, y, z, attenuation radius); kind in w of `kinds`.
+struct DevScene {
+  const DevNode* nodes;
+  const DevTri* tris;
+  const uint32_t* leafOff;   // nid.w per reference node (leaf data word offset or 0xFFFFFFFF)
+  const uint32_t* leafData;  // the reference leaf u32 stream (light lists are read from it)
+  const float* lights;       // 4 floats per light: pos.xyz, radius
+  const uint32_t* lightKind; // PackedLight.iparam per light
+  uint32_t root;             // root ref
+  uint32_t numNodes;         // reference node count
+  uint32_t leafWords;
+  uint32_t numLights;
+};
+
+}  // namespace rtq

@@ -1,0 +1,422 @@
+// C ABI of the gfx950 ray-query library (include/rtq.h): device handles, blob upload into the
+// device layout, dispatch of the traversal kernels, read-back, shadow masks and light culling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/rtq.h"
+#include "kernels.h"
+#include "layout.h"
+
+struct rtq_device {
+  int ordinal = 0;
+};
+
+struct rtq_scene {
+  rtq_device* dev = nullptr;
+  rtq::DevScene view{};
+  rtq_scene_info info{};
+  std::vector<void*> allocations;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+rtq_status fail(rtq_status s, const std::string& msg) {
+  g_error = msg;
+  return s;
+}
+
+rtq_status hipFail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? RTQ_ERR_OUT_OF_MEMORY : RTQ_ERR_HIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define RTQ_HIP(call)                              \
+  do {                                             \
+    hipError_t e_ = (call);                        \
+    if (e_ != hipSuccess) return hipFail(e_, #call); \
+  } while (0)
+
+struct TriKey {
+  uint32_t a, b, c;
+  bool operator==(const TriKey& o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct TriKeyHash {
+  size_t operator()(const TriKey& k) const {
+    uint64_t h = 1469598103934665603ull;
+    h = (h ^ k.a) * 1099511628211ull;
+    h = (h ^ k.b) * 1099511628211ull;
+    h = (h ^ k.c) * 1099511628211ull;
+    return size_t(h ^ (h >> 32));
+  }
+};
+
+template <class T>
+rtq_status uploadArray(rtq_scene* s, const std::vector<T>& host, const T** dptr) {
+  size_t bytes = std::max<size_t>(host.size() * sizeof(T), 16);
+  void* p = nullptr;
+  RTQ_HIP(hipMalloc(&p, bytes));
+  s->allocations.push_back(p);
+  if (!host.empty()) RTQ_HIP(hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+  s->info.device_bytes += bytes;
+  *dptr = static_cast<const T*>(p);
+  return RTQ_OK;
+}
+
+void freeScene(rtq_scene* s) {
+  if (!s) return;
+  if (s->dev) hipSetDevice(s->dev->ordinal);
+  for (void* p : s->allocations) hipFree(p);
+  delete s;
+}
+
+int stackClass(uint32_t depth) { return depth <= 32 ? 32 : 64; }
+
+}  // namespace
+
+extern "C" {
+
+const char* rtq_status_string(rtq_status s) {
+  switch (s) {
+    case RTQ_OK: return "RTQ_OK";
+    case RTQ_ERR_INVALID_ARGUMENT: return "RTQ_ERR_INVALID_ARGUMENT";
+    case RTQ_ERR_HIP: return "RTQ_ERR_HIP";
+    case RTQ_ERR_OUT_OF_MEMORY: return "RTQ_ERR_OUT_OF_MEMORY";
+    case RTQ_ERR_LIMIT: return "RTQ_ERR_LIMIT";
+    case RTQ_ERR_BAD_BLOB: return "RTQ_ERR_BAD_BLOB";
+    case RTQ_ERR_NO_DEVICE: return "RTQ_ERR_NO_DEVICE";
+    case RTQ_ERR_BUDGET: return "RTQ_ERR_BUDGET";
+    case RTQ_ERR_QUEUE_FULL: return "RTQ_ERR_QUEUE_FULL";
+    case RTQ_ERR_CANCELLED: return "RTQ_ERR_CANCELLED";
+    case RTQ_ERR_TIMEOUT: return "RTQ_ERR_TIMEOUT";
+    case RTQ_ERR_INTERNAL: return "RTQ_ERR_INTERNAL";
+  }
+  return "RTQ_ERR_UNKNOWN";
+}
+
+const char* rtq_last_error(void) { return g_error.c_str(); }
+
+rtq_status rtq_device_create(int hip_ordinal, rtq_device** out) {
+  if (!out) return fail(RTQ_ERR_INVALID_ARGUMENT, "out is NULL");
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(RTQ_ERR_NO_DEVICE, "no HIP device visible");
+  if (hip_ordinal < 0 || hip_ordinal >= count) return fail(RTQ_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+  RTQ_HIP(hipSetDevice(hip_ordinal));
+  hipDeviceProp_t prop;
+  RTQ_HIP(hipGetDeviceProperties(&prop, hip_ordinal));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(RTQ_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+  *out = new rtq_device{hip_ordinal};
+  return RTQ_OK;
+}
+
+rtq_status rtq_device_destroy(rtq_device* dev) {
+  delete dev;
+  return RTQ_OK;
+}
+
+rtq_status rtq_device_alloc(rtq_device* dev, size_t bytes, void** d_ptr) {
+  if (!dev || !d_ptr) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  RTQ_HIP(hipMalloc(d_ptr, std::max<size_t>(bytes, 1)));
+  return RTQ_OK;
+}
+
+rtq_status rtq_device_free(rtq_device* dev, void* d_ptr) {
+  if (!dev) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL device");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  if (d_ptr) RTQ_HIP(hipFree(d_ptr));
+  return RTQ_OK;
+}
+
+rtq_status rtq_copy_to_device(rtq_device* dev, void* d_dst, const void* h_src, size_t bytes, void* stream) {
+  if (!dev || (!d_dst && bytes) || (!h_src && bytes)) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  RTQ_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)));
+  RTQ_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return RTQ_OK;
+}
+
+rtq_status rtq_read_hits(rtq_device* dev, const void* d_src, void* h_dst, size_t bytes, void* stream) {
+  if (!dev || (!d_src && bytes) || (!h_dst && bytes)) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  RTQ_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)));
+  RTQ_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return RTQ_OK;
+}
+
+// upload-BVH: decode the fillGpuBuffer blob (BVHBuilder.cpp:1319-1534) as the shaders bind it
+// (BVHData::fillBvhBindings, BVHData.cpp:17-26) and rebuild it in the device layout.
+rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, const rtq_ranges* ranges,
+                            const float* positions, uint32_t numVertices, rtq_scene** out) {
+  if (!dev || !blobPtr || !ranges || !out || (!positions && numVertices))
+    return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  const uint8_t* blob = static_cast<const uint8_t*>(blobPtr);
+  auto inBlob = [&](const rtq_range& r) { return size_t(r.offset) + r.size <= bytes && r.offset % 4 == 0; };
+  if (!inBlob(ranges->triangles) || !inBlob(ranges->lights) || !inBlob(ranges->nodes) || !inBlob(ranges->leaf_data))
+    return fail(RTQ_ERR_BAD_BLOB, "a range points outside the blob");
+  const uint32_t numNodes = ranges->nodes.size / 64u;
+  const uint32_t leafWords = ranges->leaf_data.size / 4u;
+  const uint32_t numTris = ranges->triangles.size / 12u;
+  const uint32_t numLights = ranges->lights.size / 56u;
+  if (numNodes == 0) return fail(RTQ_ERR_BAD_BLOB, "no nodes");
+  const uint32_t* nodes = reinterpret_cast<const uint32_t*>(blob + ranges->nodes.offset);
+  const uint32_t* leaf = reinterpret_cast<const uint32_t*>(blob + ranges->leaf_data.offset);
+  const uint32_t* tris = reinterpret_cast<const uint32_t*>(blob + ranges->triangles.offset);
+  const uint32_t* lightsRaw = reinterpret_cast<const uint32_t*>(blob + ranges->lights.offset);
+
+  rtq_scene* s = new rtq_scene();
+  s->dev = dev;
+  auto bail = [&](rtq_status st) {
+    freeScene(s);
+    return st;
+  };
+
+  // triangle triple -> first triangle index (the hit record's tri_id)
+  std::unordered_map<TriKey, uint32_t, TriKeyHash> triIndex;
+  triIndex.reserve(size_t(numTris) * 2);
+  for (uint32_t i = 0; i < numTris; ++i) triIndex.emplace(TriKey{tris[3 * i], tris[3 * i + 1], tris[3 * i + 2]}, i);
+
+  // Walk the tree from the root through child pointers (bvhGetter.glsl:17-25); numbering the
+  // inner nodes in reference order keeps each sibling pair's parent record in one place.
+  std::vector<uint32_t> innerIndex(numNodes, 0xFFFFFFFFu);
+  std::vector<uint8_t> isLeaf(numNodes, 0), seen(numNodes, 0);
+  std::vector<uint32_t> leafSlot(numNodes, rtq::kRefEmpty);
+  const bool rootIsLeaf = numNodes == 1;
+  uint32_t maxDepth = 0;
+  {
+    struct Item { uint32_t id; uint32_t depth; bool leaf; };
+    std::vector<Item> st{{0u, 0u, rootIsLeaf}};
+    while (!st.empty()) {
+      Item it = st.back();
+      st.pop_back();
+      if (it.id >= numNodes) return bail(fail(RTQ_ERR_BAD_BLOB, "child id beyond node count"));
+      if (seen[it.id]) return bail(fail(RTQ_ERR_BAD_BLOB, "node reached twice"));
+      seen[it.id] = 1;
+      if (it.depth > 64) return bail(fail(RTQ_ERR_LIMIT, "tree deeper than 64 levels"));
+      if (it.leaf) {
+        isLeaf[it.id] = 1;
+        maxDepth = std::max(maxDepth, it.depth);
+        continue;
+      }
+      const uint32_t packed = nodes[16 * it.id + 2];
+      const uint32_t l = packed & 0x3FFFFFFFu;
+      if (l + 1u >= numNodes) return bail(fail(RTQ_ERR_BAD_BLOB, "child id beyond node count"));
+      st.push_back({l + 1u, it.depth + 1u, (packed & 0x40000000u) != 0});
+      st.push_back({l, it.depth + 1u, (packed & 0x80000000u) != 0});
+    }
+  }
+  std::vector<rtq::DevTri> dtris;
+  std::vector<uint32_t> innerIds;
+  for (uint32_t id = 0; id < numNodes; ++id) {
+    if (!seen[id]) continue;
+    if (!isLeaf[id]) {
+      innerIndex[id] = uint32_t(innerIds.size());
+      innerIds.push_back(id);
+      continue;
+    }
+    const uint32_t off = nodes[16 * id + 3];
+    if (off == 0xFFFFFFFFu) continue;
+    if (off >= leafWords) return bail(fail(RTQ_ERR_BAD_BLOB, "leaf data offset beyond the leaf range"));
+    const uint32_t count = leaf[off] & 0xFFFu;
+    if (count == 0) continue;
+    if (size_t(off) + 1 + 3 * size_t(count) > leafWords) return bail(fail(RTQ_ERR_BAD_BLOB, "leaf triangles overrun"));
+    leafSlot[id] = rtq::kRefLeaf | uint32_t(dtris.size());
+    for (uint32_t k = 0; k < count; ++k) {
+      const uint32_t* t = leaf + off + 1 + 3 * k;
+      uint32_t vi[3] = {t[0] + (t[1] & 0xFFFFu), t[0] + (t[1] >> 16), t[0] + (t[2] & 0xFFFFu)};
+      float p[3][3];
+      for (int q = 0; q < 3; ++q) {
+        if (vi[q] >= numVertices) return bail(fail(RTQ_ERR_BAD_BLOB, "triangle vertex beyond position count"));
+        p[q][0] = positions[3 * size_t(vi[q])];
+        p[q][1] = positions[3 * size_t(vi[q]) + 1];
+        p[q][2] = positions[3 * size_t(vi[q]) + 2];
+      }
+      auto found = triIndex.find(TriKey{t[0], t[1], t[2]});
+      const uint32_t triId = found == triIndex.end() ? RTQ_NO_HIT : found->second;
+      rtq::DevTri d{};
+      const float e1[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+      const float e2[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+      d.t0[0] = p[0][0]; d.t0[1] = p[0][1]; d.t0[2] = p[0][2]; d.t0[3] = e1[0];
+      d.t1[0] = e1[1]; d.t1[1] = e1[2]; d.t1[2] = e2[0]; d.t1[3] = e2[1];
+      uint32_t w[4] = {0, triId, id, k + 1 == count ? 1u : 0u};
+      std::memcpy(&d.t2, w, sizeof(w));
+      d.t2[0] = e2[2];
+      dtris.push_back(d);
+    }
+  }
+  auto ref = [&](uint32_t id) { return isLeaf[id] ? leafSlot[id] : innerIndex[id]; };
+  std::vector<rtq::DevNode> dnodes(innerIds.size());
+  for (size_t k = 0; k < innerIds.size(); ++k) {
+    const uint32_t id = innerIds[k];
+    const uint32_t* n = nodes + 16 * id;
+    const float* f = reinterpret_cast<const float*>(n);
+    const float* n0xy = f + 4;
+    const float* n1xy = f + 8;
+    const float* nz = f + 12;
+    rtq::DevNode& d = dnodes[k];
+    d.a[0] = n0xy[0]; d.a[1] = n0xy[1]; d.a[2] = nz[0]; d.a[3] = n0xy[2];
+    d.b[0] = n0xy[3]; d.b[1] = nz[1]; d.b[2] = n1xy[0]; d.b[3] = n1xy[1];
+    d.c[0] = nz[2]; d.c[1] = n1xy[2]; d.c[2] = n1xy[3]; d.c[3] = nz[3];
+    const uint32_t l = n[2] & 0x3FFFFFFFu;
+    d.d[0] = ref(l);
+    d.d[1] = ref(l + 1);
+    d.d[2] = id;
+    d.d[3] = 0;
+  }
+  std::vector<uint32_t> leafOff(numNodes);
+  for (uint32_t id = 0; id < numNodes; ++id) leafOff[id] = nodes[16 * id + 3];
+  std::vector<uint32_t> leafData(leaf, leaf + leafWords);
+  std::vector<float> lights(size_t(numLights) * 4);
+  std::vector<uint32_t> kinds(numLights);
+  for (uint32_t i = 0; i < numLights; ++i) {
+    const uint32_t* L = lightsRaw + 14 * size_t(i);
+    kinds[i] = L[0];
+    std::memcpy(&lights[4 * size_t(i)], L + 1, 16);
+  }
+
+  hipError_t e = hipSetDevice(dev->ordinal);
+  if (e != hipSuccess) return bail(hipFail(e, "hipSetDevice"));
+  rtq_status st;
+  if ((st = uploadArray(s, dnodes, &s->view.nodes)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, dtris, &s->view.tris)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, leafOff, &s->view.leafOff)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, leafData, &s->view.leafData)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, lights, &s->view.lights)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, kinds, &s->view.lightKind)) != RTQ_OK) return bail(st);
+  s->view.root = rootIsLeaf ? leafSlot[0] : 0u;
+  s->view.numNodes = numNodes;
+  s->view.leafWords = leafWords;
+  s->view.numLights = numLights;
+  s->info.num_nodes = numNodes;
+  s->info.num_inner = uint32_t(innerIds.size());
+  s->info.num_leaf_slots = uint32_t(dtris.size());
+  s->info.num_triangles = numTris;
+  s->info.num_lights = numLights;
+  s->info.max_depth = maxDepth;
+  *out = s;
+  return RTQ_OK;
+}
+
+rtq_status rtq_scene_destroy(rtq_scene* scene) {
+  freeScene(scene);
+  return RTQ_OK;
+}
+
+rtq_status rtq_scene_get_info(const rtq_scene* scene, rtq_scene_info* info) {
+  if (!scene || !info) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  *info = scene->info;
+  return RTQ_OK;
+}
+
+static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32_t flags, void* out, uint32_t* nid,
+                           uint32_t* counters, void* stream) {
+  if (!s) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene");
+  if (n == 0) return RTQ_OK;
+  if (!rays || !out) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL ray or output buffer");
+  if (flags != RTQ_TRACE_CLOSEST && flags != RTQ_TRACE_ANY) return fail(RTQ_ERR_INVALID_ARGUMENT, "bad flags");
+  RTQ_HIP(hipSetDevice(s->dev->ordinal));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
+  const float4* r = reinterpret_cast<const float4*>(rays);
+  const bool deep = stackClass(s->info.max_depth) == 64;
+  if (flags == RTQ_TRACE_CLOSEST) {
+    float4* h = static_cast<float4*>(out);
+    if (counters) {
+      if (deep) hipLaunchKernelGGL((rtq::kTraceClosest<64, true>), grid, block, 0, st, s->view, r, n, h, nid, counters);
+      else hipLaunchKernelGGL((rtq::kTraceClosest<32, true>), grid, block, 0, st, s->view, r, n, h, nid, counters);
+    } else {
+      if (deep) hipLaunchKernelGGL((rtq::kTraceClosest<64, false>), grid, block, 0, st, s->view, r, n, h, nid, nullptr);
+      else hipLaunchKernelGGL((rtq::kTraceClosest<32, false>), grid, block, 0, st, s->view, r, n, h, nid, nullptr);
+    }
+  } else {
+    uint32_t* o = static_cast<uint32_t*>(out);
+    if (counters) {
+      if (deep) hipLaunchKernelGGL((rtq::kTraceAny<64, true>), grid, block, 0, st, s->view, r, n, o, counters);
+      else hipLaunchKernelGGL((rtq::kTraceAny<32, true>), grid, block, 0, st, s->view, r, n, o, counters);
+    } else {
+      if (deep) hipLaunchKernelGGL((rtq::kTraceAny<64, false>), grid, block, 0, st, s->view, r, n, o, nullptr);
+      else hipLaunchKernelGGL((rtq::kTraceAny<32, false>), grid, block, 0, st, s->view, r, n, o, nullptr);
+    }
+  }
+  RTQ_HIP(hipGetLastError());
+  return RTQ_OK;
+}
+
+rtq_status rtq_dispatch_rays(rtq_scene* scene, const rtq_ray* d_rays, uint32_t num_rays, uint32_t flags, void* d_out,
+                             uint32_t* d_leaf_nid, void* stream) {
+  return dispatch(scene, d_rays, num_rays, flags, d_out, flags == RTQ_TRACE_CLOSEST ? d_leaf_nid : nullptr, nullptr,
+                  stream);
+}
+
+rtq_status rtq_dispatch_rays_counted(rtq_scene* scene, const rtq_ray* d_rays, uint32_t num_rays, uint32_t flags,
+                                     void* d_out, uint32_t* d_leaf_nid, uint32_t* d_counters, void* stream) {
+  if (!d_counters && num_rays) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL counters");
+  return dispatch(scene, d_rays, num_rays, flags, d_out, flags == RTQ_TRACE_CLOSEST ? d_leaf_nid : nullptr,
+                  d_counters, stream);
+}
+
+rtq_status rtq_shadow_masks(rtq_scene* s, const rtq_ray* d_rays, const rtq_hit* d_hits, const uint32_t* d_leaf_nid,
+                            uint32_t n, const float sun_dir[3], uint32_t flags, uint32_t* d_masks, void* stream) {
+  if (!s) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene");
+  if (n == 0) return RTQ_OK;
+  if (!d_rays || !d_hits || !d_leaf_nid || !d_masks) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL buffer");
+  if ((flags & RTQ_SHADOW_SUN) && !sun_dir) return fail(RTQ_ERR_INVALID_ARGUMENT, "sun flag without direction");
+  float sd[3] = {0, 0, 0};
+  if (sun_dir) std::memcpy(sd, sun_dir, sizeof(sd));
+  RTQ_HIP(hipSetDevice(s->dev->ordinal));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
+  const float4* r = reinterpret_cast<const float4*>(d_rays);
+  const float4* h = reinterpret_cast<const float4*>(d_hits);
+  if (stackClass(s->info.max_depth) == 64)
+    hipLaunchKernelGGL((rtq::kShadowMasks<64>), grid, block, 0, st, s->view, r, h, d_leaf_nid, n, sd[0], sd[1], sd[2],
+                       flags, d_masks);
+  else
+    hipLaunchKernelGGL((rtq::kShadowMasks<32>), grid, block, 0, st, s->view, r, h, d_leaf_nid, n, sd[0], sd[1], sd[2],
+                       flags, d_masks);
+  RTQ_HIP(hipGetLastError());
+  return RTQ_OK;
+}
+
+rtq_status rtq_cull_lights(rtq_scene* s, const rtq_frustum* f, uint32_t width, uint32_t height,
+                           uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity, uint32_t* h_total,
+                           void* stream) {
+  if (!s || !f || !d_tile_offsets || (!d_tile_lights && capacity)) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (width == 0 || height == 0) return fail(RTQ_ERR_INVALID_ARGUMENT, "empty frame");
+  const uint32_t ngx = (width + 15u) / 16u, ngy = (height + 15u) / 16u;
+  const uint64_t ntiles64 = uint64_t(ngx) * ngy;
+  if (ntiles64 > (1u << 24)) return fail(RTQ_ERR_LIMIT, "more than 2^24 tiles");
+  const uint32_t ntiles = uint32_t(ntiles64);
+  RTQ_HIP(hipSetDevice(s->dev->ordinal));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  rtq::Frustum4 fr;
+  std::memcpy(&fr, f, sizeof(fr));
+  const float4* lights = reinterpret_cast<const float4*>(s->view.lights);
+  uint32_t* counts = nullptr;
+  RTQ_HIP(hipMallocAsync(reinterpret_cast<void**>(&counts), sizeof(uint32_t) * ntiles, st));
+  hipLaunchKernelGGL(rtq::kCullCount, dim3(ntiles), dim3(rtq::kCullBlock), 0, st, fr, lights, s->view.numLights, ngx,
+                     ngy, counts);
+  hipLaunchKernelGGL(rtq::kScan, dim3(1), dim3(1024), 0, st, counts, ntiles, d_tile_offsets);
+  hipLaunchKernelGGL(rtq::kCullFill, dim3(ntiles), dim3(rtq::kCullBlock), 0, st, fr, lights, s->view.numLights, ngx,
+                     ngy, d_tile_offsets, d_tile_lights, capacity);
+  hipError_t le = hipGetLastError();
+  uint32_t total = 0;
+  hipMemcpyAsync(&total, d_tile_offsets + ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+  hipFreeAsync(counts, st);
+  RTQ_HIP(hipStreamSynchronize(st));
+  if (le != hipSuccess) return hipFail(le, "culling kernels");
+  if (h_total) *h_total = total;
+  if (total > capacity) return fail(RTQ_ERR_LIMIT, "tile light lists exceed capacity");
+  return RTQ_OK;
+}
+
+}  // extern "C"

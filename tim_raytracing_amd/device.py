@@ -1,0 +1,170 @@
+"""Python view of the gfx950 library (librtq.so) over its C ABI.
+
+Mirrors the reference's upload / dispatch surface: Scene.upload stands for BVHData::build's
+CreateBuffer + UploadBuffer (Renderer/BVHData.cpp:43-48), Scene.dispatch for RayTracingPass::tracePass
+-> IRenderContext::Dispatch (Renderer/raytracingPass.cpp:89-110). There is no CPU fallback: every
+call goes to the HIP library and raises if it is missing or fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import bindings as B
+
+
+class DeviceBuffer:
+    def __init__(self, dev: "Device", nbytes: int):
+        self.dev = dev
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        B.check_device("rtq_device_alloc", dev.lib.rtq_device_alloc(dev.handle, max(1, self.nbytes), C.byref(p)))
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr is not None:
+            self.dev.lib.rtq_device_free(self.dev.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def write(self, arr: np.ndarray, stream=None):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        B.check_device("rtq_copy_to_device",
+                       self.dev.lib.rtq_copy_to_device(self.dev.handle, self.ptr, a.ctypes.data, a.nbytes, stream))
+
+    def read(self, dtype, count: int, stream=None) -> np.ndarray:
+        out = np.zeros(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        B.check_device("rtq_read_hits",
+                       self.dev.lib.rtq_read_hits(self.dev.handle, self.ptr, out.ctypes.data, out.nbytes, stream))
+        return out
+
+
+class Device:
+    def __init__(self, ordinal: int = 0):
+        self.lib = B.device_lib()
+        h = C.c_void_p()
+        B.check_device("rtq_device_create", self.lib.rtq_device_create(ordinal, C.byref(h)))
+        self.handle = h
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr: np.ndarray) -> DeviceBuffer:
+        buf = DeviceBuffer(self, np.asarray(arr).nbytes)
+        buf.write(arr)
+        return buf
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            self.lib.rtq_device_destroy(h)
+            self.handle = None
+
+
+class Scene:
+    def __init__(self, dev: Device, blob: np.ndarray, ranges, positions: np.ndarray):
+        self.dev = dev
+        blob = np.ascontiguousarray(blob, np.uint8)
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        h = C.c_void_p()
+        B.check_device("rtq_scene_upload",
+                       dev.lib.rtq_scene_upload(dev.handle, blob.ctypes.data, blob.size, C.byref(ranges),
+                                                pos.ctypes.data, pos.size // 3, C.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def from_builder(cls, dev: Device, builder) -> "Scene":
+        blob, ranges = builder.fill_gpu_buffer()
+        return cls(dev, blob, ranges, builder.positions())
+
+    def info(self) -> B.SceneInfo:
+        i = B.SceneInfo()
+        B.check_device("rtq_scene_get_info", self.dev.lib.rtq_scene_get_info(self.handle, C.byref(i)))
+        return i
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            self.dev.lib.rtq_scene_destroy(h)
+            self.handle = None
+
+    # dispatch-rays on device buffers (asynchronous on `stream`)
+    def dispatch(self, d_rays: int, n: int, flags: int, d_out: int, d_nid: int | None = None, stream=None):
+        B.check_device("rtq_dispatch_rays",
+                       self.dev.lib.rtq_dispatch_rays(self.handle, d_rays, n, flags, d_out, d_nid, stream))
+
+    def dispatch_counted(self, d_rays: int, n: int, flags: int, d_out: int, d_nid, d_counters: int, stream=None):
+        B.check_device("rtq_dispatch_rays_counted",
+                       self.dev.lib.rtq_dispatch_rays_counted(self.handle, d_rays, n, flags, d_out, d_nid,
+                                                              d_counters, stream))
+
+    def shadow_masks(self, d_rays, d_hits, d_nid, n, sun_dir, flags, d_masks, stream=None):
+        sd = np.asarray(sun_dir if sun_dir is not None else (0, 0, 0), np.float32)
+        B.check_device("rtq_shadow_masks",
+                       self.dev.lib.rtq_shadow_masks(self.handle, d_rays, d_hits, d_nid, n, sd.ctypes.data, flags,
+                                                     d_masks, stream))
+
+    # ---- host-array conveniences (allocate, copy, run, read back) ----
+    def trace_closest(self, rays: np.ndarray, counted: bool = False):
+        n = rays.size
+        d_r = self.dev.upload(rays)
+        d_h = self.dev.alloc(16 * max(n, 1))
+        d_n = self.dev.alloc(4 * max(n, 1))
+        out = {}
+        if counted:
+            d_c = self.dev.alloc(12 * max(n, 1))
+            self.dispatch_counted(d_r.ptr, n, B.RTQ_TRACE_CLOSEST, d_h.ptr, d_n.ptr, d_c.ptr)
+            out["counters"] = d_c.read(np.uint32, 3 * n).reshape(n, 3)
+        else:
+            self.dispatch(d_r.ptr, n, B.RTQ_TRACE_CLOSEST, d_h.ptr, d_n.ptr)
+        hits = d_h.read(B.HIT_DTYPE, n)
+        out.update(tri_id=hits["tri_id"].copy(), t=hits["t"].copy(), u=hits["u"].copy(), v=hits["v"].copy(),
+                   nid=d_n.read(np.uint32, n), hits=hits)
+        return out
+
+    def trace_any(self, rays: np.ndarray, counted: bool = False):
+        n = rays.size
+        d_r = self.dev.upload(rays)
+        d_o = self.dev.alloc(4 * max(n, 1))
+        if counted:
+            d_c = self.dev.alloc(12 * max(n, 1))
+            self.dispatch_counted(d_r.ptr, n, B.RTQ_TRACE_ANY, d_o.ptr, None, d_c.ptr)
+            return d_o.read(np.uint32, n), d_c.read(np.uint32, 3 * n).reshape(n, 3)
+        self.dispatch(d_r.ptr, n, B.RTQ_TRACE_ANY, d_o.ptr, None)
+        return d_o.read(np.uint32, n)
+
+    def shadow(self, rays: np.ndarray, hits: np.ndarray, nid: np.ndarray, sun_dir, flags: int) -> np.ndarray:
+        n = rays.size
+        d_r = self.dev.upload(rays)
+        d_h = self.dev.upload(hits)
+        d_n = self.dev.upload(np.ascontiguousarray(nid, np.uint32))
+        d_m = self.dev.alloc(4 * max(n, 1))
+        self.shadow_masks(d_r.ptr, d_h.ptr, d_n.ptr, n, sun_dir, flags, d_m.ptr)
+        return d_m.read(np.uint32, n)
+
+    def cull_lights(self, frustum: B.Frustum, width: int, height: int, capacity: int | None = None):
+        ntiles = ((width + 15) // 16) * ((height + 15) // 16)
+        d_off = self.dev.alloc(4 * (ntiles + 1))
+        total = C.c_uint32()
+        if capacity is None:
+            # first pass with no list storage gives the total
+            st = self.dev.lib.rtq_cull_lights(self.handle, C.byref(frustum), width, height, d_off.ptr, None, 0,
+                                              C.byref(total), None)
+            if st not in (B.RTQ_OK, 4):
+                B.check_device("rtq_cull_lights", st)
+            capacity = total.value
+        d_l = self.dev.alloc(4 * max(1, capacity))
+        B.check_device("rtq_cull_lights",
+                       self.dev.lib.rtq_cull_lights(self.handle, C.byref(frustum), width, height, d_off.ptr, d_l.ptr,
+                                                    capacity, C.byref(total), None))
+        offs = d_off.read(np.uint32, ntiles + 1)
+        lists = d_l.read(np.uint32, max(1, total.value))[:total.value]
+        return offs, lists

@@ -1,0 +1,177 @@
+"""Python view of the host builder and workloads (librtq_build.so), used by tests and bench.py.
+
+Method names follow the reference classes they stand for: BVHGeometry::addTriangleList,
+BVHBuilder::addTriangleList / addSphereLight / setParameters / build / getBvhGpuSize /
+fillGpuBuffer (Renderer/BVHBuilder.h:63-101, Renderer/BVHGeometry.h:17-22).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import bindings as B
+
+# fov of the reference camera pass: 70 * 3.14 / 180 in fp32 (Renderer/raytracingPass.cpp:43)
+REFERENCE_FOV = float(np.float32(70.0) * np.float32(3.14) / np.float32(180.0))
+TMAX = 100.0
+
+
+class BVHBuilder:
+    def __init__(self):
+        self._lib = B.build_lib()
+        h = C.c_void_p()
+        B.check_build("rtq_builder_create", self._lib.rtq_builder_create(C.byref(h)))
+        self._h = h
+        self.built = False
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.rtq_builder_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    # BVHGeometry::addTriangleList
+    def add_vertices(self, xyz) -> int:
+        a = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        off = C.c_uint32()
+        B.check_build("rtq_builder_add_vertices",
+                      self._lib.rtq_builder_add_vertices(self._h, B.ptr(a), a.shape[0], C.byref(off)))
+        self.built = False
+        return off.value
+
+    # BVHBuilder::addTriangleList
+    def add_triangle_list(self, vertex_offset: int, indices, color=(0.7, 0.7, 0.7)):
+        idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+        assert idx.size % 3 == 0
+        col = np.asarray(color, dtype=np.float32)
+        B.check_build("rtq_builder_add_triangle_list",
+                      self._lib.rtq_builder_add_triangle_list(self._h, vertex_offset, idx.size // 3,
+                                                              B.ptr(idx), B.ptr(col)))
+        self.built = False
+
+    def add_sphere_light(self, pos, radius, color=(1, 1, 1), sphere_radius=0.1):
+        p = np.asarray(pos, dtype=np.float32)
+        c = np.asarray(color, dtype=np.float32)
+        B.check_build("rtq_builder_add_sphere_light",
+                      self._lib.rtq_builder_add_sphere_light(self._h, B.ptr(p), radius, B.ptr(c), sphere_radius))
+        self.built = False
+
+    def set_parameters(self, max_depth=25, min_obj_per_node=4, min_obj_gain=4,
+                       expand_node_volume_threshold=0.25, expand_node_dimension_factor=0.5):
+        bp = B.BuildParams(max_depth, min_obj_per_node, min_obj_gain,
+                           expand_node_volume_threshold, expand_node_dimension_factor)
+        B.check_build("rtq_builder_set_parameters", self._lib.rtq_builder_set_parameters(self._h, C.byref(bp)))
+        self.built = False
+
+    def generate(self, kind: int, seed: int, param: int = 0):
+        B.check_build("rtq_scene_generate", self._lib.rtq_scene_generate(self._h, kind, seed, param))
+        self.built = False
+
+    def generate_lights(self, seed: int, count: int, lo=-100.0, hi=100.0, rmin=0.5, rmax=20.0):
+        B.check_build("rtq_lights_generate",
+                      self._lib.rtq_lights_generate(self._h, seed, count, lo, hi, rmin, rmax))
+        self.built = False
+
+    def build(self, threads: int = 1):
+        B.check_build("rtq_builder_build", self._lib.rtq_builder_build(self._h, threads))
+        self.built = True
+        return self.stats()
+
+    def stats(self) -> B.BuildStats:
+        s = B.BuildStats()
+        B.check_build("rtq_builder_get_stats", self._lib.rtq_builder_get_stats(self._h, C.byref(s)))
+        return s
+
+    def gpu_size(self) -> int:
+        n = C.c_size_t()
+        B.check_build("rtq_builder_gpu_size", self._lib.rtq_builder_gpu_size(self._h, C.byref(n)))
+        return n.value
+
+    def fill_gpu_buffer(self):
+        """-> (blob as uint8 array, Ranges)"""
+        blob = np.zeros(self.gpu_size(), dtype=np.uint8)
+        r = B.Ranges()
+        B.check_build("rtq_builder_fill_gpu_buffer",
+                      self._lib.rtq_builder_fill_gpu_buffer(self._h, B.ptr(blob), blob.size, C.byref(r)))
+        return blob, r
+
+    def positions(self) -> np.ndarray:
+        p = C.c_void_p()
+        n = C.c_uint32()
+        B.check_build("rtq_builder_positions", self._lib.rtq_builder_positions(self._h, C.byref(p), C.byref(n)))
+        if n.value == 0:
+            return np.zeros((0, 3), np.float32)
+        buf = (C.c_float * (3 * n.value)).from_address(p.value)
+        return np.frombuffer(buf, dtype=np.float32).reshape(-1, 3).copy()
+
+    def scene_box(self):
+        mn = np.zeros(3, np.float32)
+        mx = np.zeros(3, np.float32)
+        B.check_build("rtq_builder_scene_box", self._lib.rtq_builder_scene_box(self._h, B.ptr(mn), B.ptr(mx)))
+        return mn, mx
+
+    def surface_rays(self, seed: int, count: int, tmax: float = TMAX) -> np.ndarray:
+        out = np.zeros(count, dtype=B.RAY_DTYPE)
+        B.check_build("rtq_surface_rays", self._lib.rtq_surface_rays(self._h, seed, count, tmax, B.ptr(out)))
+        return out
+
+    def bounce_rays(self, seed: int, rays: np.ndarray, hits: np.ndarray, tmax: float = TMAX) -> np.ndarray:
+        assert rays.dtype == B.RAY_DTYPE and hits.dtype == B.HIT_DTYPE and rays.size == hits.size
+        out = np.zeros(rays.size, dtype=B.RAY_DTYPE)
+        B.check_build("rtq_bounce_rays", self._lib.rtq_bounce_rays(self._h, seed, B.ptr(rays), B.ptr(hits),
+                                                                   rays.size, tmax, B.ptr(out)))
+        return out
+
+
+def camera_frustum(pos, direction, up=(0, 0, 1), fov_y=REFERENCE_FOV, aspect=1.0) -> B.Frustum:
+    lib = B.build_lib()
+    p = np.asarray(pos, np.float32)
+    d = np.asarray(direction, np.float32)
+    u = np.asarray(up, np.float32)
+    f = B.Frustum()
+    B.check_build("rtq_camera_frustum", lib.rtq_camera_frustum(B.ptr(p), B.ptr(d), B.ptr(u), fov_y, aspect,
+                                                               C.byref(f)))
+    return f
+
+
+def frustum_rays(f: B.Frustum, width: int, height: int, seed: int, tmax: float = TMAX) -> np.ndarray:
+    lib = B.build_lib()
+    out = np.zeros(width * height, dtype=B.RAY_DTYPE)
+    B.check_build("rtq_frustum_rays", lib.rtq_frustum_rays(C.byref(f), width, height, seed, tmax, B.ptr(out)))
+    return out
+
+
+def workload(config: int, scale: float = 1.0, threads: int = 8):
+    """Build one of the benchmark scenes. scale < 1 shrinks the triangle/light count for tests.
+
+    config 1: closed box, 16 384 tris; 256x256 primary rays from (5,1,5) looking +y.
+    config 2: 4096 UV spheres x 256 = 1 048 576 tris; 1920x1080 primary rays from the origin
+              looking +x; one point light at (0,0,150) with radius 1e4.
+    config 3: 3 Mi-triangle Gaussian-mixture soup with 4096 point lights.
+    Returns (builder, frustum, width, height).
+    """
+    b = BVHBuilder()
+    if config == 1:
+        b.generate(B.RTQ_SCENE_BOX, 1, 0)
+        w = h = 256
+        f = camera_frustum((5, 1, 5), (0, 1, 0), (0, 0, 1), REFERENCE_FOV, 1.0)
+    elif config == 2:
+        b.generate(B.RTQ_SCENE_SPHERES, 2, max(1, int(round(4096 * scale))))
+        b.add_sphere_light((0, 0, 150), 1e4, (1, 1, 1), 0.1)
+        w, h = 1920, 1080
+        f = camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), REFERENCE_FOV, w / h)
+    elif config == 3:
+        b.generate(B.RTQ_SCENE_SOUP, 3, max(1, int(round(3 * 1024 * 1024 * scale))))
+        b.generate_lights(3, max(1, int(round(4096 * min(1.0, scale * 4)))), -100, 100, 0.5, 20)
+        w, h = 1920, 1080
+        f = camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), REFERENCE_FOV, w / h)
+    else:
+        raise ValueError(config)
+    b.build(threads)
+    return b, f, w, h

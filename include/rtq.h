@@ -136,6 +136,14 @@ rtq_status rtq_dispatch_rays(rtq_scene* scene, const rtq_ray* d_rays, uint32_t n
 rtq_status rtq_dispatch_rays_counted(rtq_scene* scene, const rtq_ray* d_rays, uint32_t num_rays, uint32_t flags,
                                      void* d_out, uint32_t* d_leaf_nid, uint32_t* d_counters, void* stream);
 
+#define RTQ_CAMERA_REFERENCE_ROWS 1u /* store at the reference's row 16*ceil(H/16) - gy; row 0 untouched */
+
+/* dispatch-rays with rays generated in the kernel, as RayTracingPass::tracePass does: one primary
+ * ray per pixel from the frustum (cameraPass.comp:46-56), t limit TMAX = 100, closest hit.
+ * d_hits / d_leaf_nid hold width*height records, pixel (x, y) at y*width + x, y = 0 at corner00. */
+rtq_status rtq_trace_camera(rtq_scene* scene, const rtq_frustum* frustum, uint32_t width, uint32_t height,
+                            uint32_t flags, rtq_hit* d_hits, uint32_t* d_leaf_nid, void* stream);
+
 /* read-hits: synchronising copy of `bytes` from device memory written by a dispatch on `stream`. */
 rtq_status rtq_read_hits(rtq_device* dev, const void* d_src, void* h_dst, size_t bytes, void* stream);
 
@@ -159,6 +167,11 @@ rtq_status rtq_shadow_masks(rtq_scene* scene, const rtq_ray* d_rays, const rtq_h
 rtq_status rtq_cull_lights(rtq_scene* scene, const rtq_frustum* frustum, uint32_t width, uint32_t height,
                            uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity, uint32_t* h_total,
                            void* stream);
+
+/* Diagnostic: compares the traversal's reciprocal-based slab division with IEEE division on
+ * `count` random operand pairs from the range where the kernels use it; *mismatches receives the
+ * number of bitwise differences (0 expected). No counterpart in the reference. Synchronous. */
+rtq_status rtq_selftest_division(rtq_device* dev, uint64_t seed, uint64_t count, uint64_t* mismatches);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,7 @@ _SIGS = {
     "orc_shadow_masks": ([P, C.c_size_t, P, P, U32, P, P, P, U32, P, U32, P, C.c_int], C.c_int),
     "orc_tile_planes": ([P, U32, U32, U32, U32, P], None),
     "orc_cull_lights": ([P, U32, P, U32, U32, P, P, C.c_uint64], C.c_uint64),
+    "orc_camera_rays": ([P, U32, U32, C.c_int, P], None),
 }
 
 _cache = {}
@@ -160,6 +161,14 @@ def cull_lights(lights: np.ndarray, frustum, width: int, height: int):
     lists = np.zeros(max(1, total), np.uint32)
     lib().orc_cull_lights(_p(L), L.shape[0], _p(fr), width, height, _p(offs), _p(lists), total)
     return offs, lists[:total]
+
+
+def camera_rays(frustum, width: int, height: int, reference_rows: bool = False) -> np.ndarray:
+    """Primary rays exactly as the camera pass generates them (rtq_ray records, y*W + x)."""
+    fr = frustum_array(frustum)
+    out = np.zeros(width * height * 8, np.float32)
+    lib().orc_camera_rays(_p(fr), width, height, int(reference_rows), _p(out))
+    return out.view(np.dtype([("org", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("pad", "<u4")]))
 
 
 def tile_planes(frustum, width, height, tx, ty) -> np.ndarray:

@@ -599,6 +599,42 @@ int orc_shadow_masks(const uint8_t* blob, size_t bytes, const u32* ranges, const
 }
 
 /* ---------------------------------------------------------------------------------------------
+ * Primary rays of the camera pass (Shaders/cameraPass.comp:46-54): stepW = (c10 - c00) * invW,
+ * invW = 1.f / W (raytracingPass.cpp:47), target = c00 + stepW * gx + stepH * gy,
+ * dir = normalize(target - cam) with normalize(v) = v / sqrt(dot(v, v)); t limit TMAX.
+ * reference_rows = 1: the ray for stored row y (>= 1) is generated from gy = 16*ceil(H/16) - y;
+ * row 0 gets tmax = -1 (the reference never writes it). Output: rtq_ray records, y*W + x. */
+void orc_camera_rays(const float* frustum, u32 width, u32 height, int reference_rows, float* out) {
+  const float* cam = frustum;
+  const float* c00 = frustum + 3;
+  const float* c10 = frustum + 6;
+  const float* c01 = frustum + 9;
+  const float invW = 1.0f / (float)width, invH = 1.0f / (float)height;
+  for (u32 y = 0; y < height; ++y)
+    for (u32 x = 0; x < width; ++x) {
+      float* r = out + 8 * ((size_t)y * width + x);
+      u32 gy = y;
+      if (reference_rows) gy = 16u * ((height + 15u) / 16u) - y;
+      float d[3];
+      for (int k = 0; k < 3; ++k) {
+        float sw = (c10[k] - c00[k]) * invW;
+        float sh = (c01[k] - c00[k]) * invH;
+        float target = (c00[k] + sw * (float)x) + sh * (float)gy;
+        d[k] = target - cam[k];
+      }
+      float len = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+      r[0] = cam[0];
+      r[1] = cam[1];
+      r[2] = cam[2];
+      r[3] = (reference_rows && y == 0) ? -1.0f : TMAX;
+      r[4] = d[0] / len;
+      r[5] = d[1] / len;
+      r[6] = d[2] / len;
+      r[7] = 0.0f;
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------
  * Tile frustum light culling: the light part of cullWithFrustumTile (Shaders/cullFrustum.glsl:
  * 63-100, 136-153) with computePlan (:9-13) and sphereFrustum4Collision (collision.glsl:219-227).
  * normalize(v) = v / sqrt(dot(v, v)). Output: full ascending list per tile (CSR). */

@@ -19,6 +19,18 @@ def _print(msg):
     print(msg, flush=True)
 
 
+def test_fast_slab_division_matches_ieee():
+    _print("case: reciprocal-based slab division vs IEEE x/d on 2^33 operand pairs")
+    import ctypes as C
+    lib = B.device_lib()
+    total = 0
+    for seed in (1, 2):
+        m = C.c_uint64()
+        B.check_device("rtq_selftest_division", lib.rtq_selftest_division(G.dev().handle, seed, 1 << 32, C.byref(m)))
+        total += m.value
+    assert total == 0
+
+
 def test_box_scene_closest_and_counters():
     _print("case: closed box, 16384 tris, 256x256 jittered primary rays")
     s = G.scene(1)
@@ -31,6 +43,23 @@ def test_box_scene_closest_and_counters():
     # the uncounted kernel gives the same records
     g2 = s["gpu"].trace_closest(rays)
     G.assert_closest_equal(g2, o, "box uncounted")
+
+
+def test_camera_pass_rays_generated_in_kernel():
+    _print("case: camera pass, rays generated in-kernel, 400x300 box scene + spheres cut, both row modes")
+    for cfg, scale, w, h in ((1, 1.0, 400, 300), (2, 1 / 16, 333, 200)):
+        s = G.scene(cfg, scale)
+        f = H.camera_frustum((5, 1, 5), (0, 1, 0), (0, 0, 1), H.REFERENCE_FOV, w / h) if cfg == 1 else \
+            H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, w / h)
+        for ref_rows in (0, 1):
+            g = s["gpu"].trace_camera(f, w, h, flags=ref_rows)
+            rays = oracle.camera_rays(f, w, h, reference_rows=bool(ref_rows))
+            sel = slice(w, None) if ref_rows else slice(None)
+            o = s["orc"].trace_closest(np.ascontiguousarray(rays[sel]), threads=16)
+            G.assert_closest_equal({k: g[k][sel] for k in ("tri_id", "nid", "t", "u", "v")}, o,
+                                   f"camera cfg{cfg} rows{ref_rows}")
+            if ref_rows:
+                assert (g["tri_id"][:w] == B.RTQ_NO_HIT).all()  # row 0 is never written
 
 
 def test_box_scene_any_hit():

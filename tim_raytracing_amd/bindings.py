@@ -25,6 +25,7 @@ STATUS_NAMES = {
 RTQ_TRACE_CLOSEST = 1
 RTQ_TRACE_ANY = 2
 RTQ_SHADOW_SUN = 1
+RTQ_CAMERA_REFERENCE_ROWS = 1
 RTQ_NO_HIT = 0xFFFFFFFF
 RTQ_SCENE_BOX, RTQ_SCENE_SPHERES, RTQ_SCENE_SOUP = 1, 2, 3
 
@@ -106,8 +107,10 @@ _DEVICE_SIGS = {
     "rtq_dispatch_rays": ([P, P, U32, U32, P, P, P], C.c_int),
     "rtq_dispatch_rays_counted": ([P, P, U32, U32, P, P, P, P], C.c_int),
     "rtq_read_hits": ([P, P, P, C.c_size_t, P], C.c_int),
+    "rtq_trace_camera": ([P, C.POINTER(Frustum), U32, U32, U32, P, P, P], C.c_int),
     "rtq_shadow_masks": ([P, P, P, P, U32, P, U32, P, P], C.c_int),
     "rtq_cull_lights": ([P, C.POINTER(Frustum), U32, U32, P, P, U32, C.POINTER(U32), P], C.c_int),
+    "rtq_selftest_division": ([P, U64, U64, C.POINTER(U64)], C.c_int),
 }
 
 

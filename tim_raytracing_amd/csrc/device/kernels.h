@@ -47,6 +47,56 @@ __device__ __forceinline__ float boxHit(const Ray3& r, float mnx, float mny, flo
   return (tmin < tm) ? tmin : -1.0f;
 }
 
+// ---- slab division without the per-quotient reciprocal -----------------------------------
+// IEEE fp32 x / d compiles to v_div_scale (x2), v_rcp, fma, fma, mul, fma, fma, fma, v_div_fmas,
+// v_div_fixup. When no operand needs rescaling (v_div_scale returns its input, VCC = 0) and x is
+// finite and non-zero, v_div_fmas is a plain fma and v_div_fixup returns the fma result, so the
+// quotient is fully determined by the refined reciprocal r(d) = fma(fma(-d, rcp(d), 1), rcp(d),
+// rcp(d)) and five fma-class operations. r(d) depends only on the ray, so it is computed once per
+// ray. Conditions that keep v_div_scale an identity: |d| in [2^-30, 2^30] and every non-zero
+// x = bound - origin in [2^-60, 2^60]; the latter holds when every box bound and origin coordinate
+// is 0 or has magnitude in [2^-37, 2^59] (differences of such values are 0 or >= 2^-60). For
+// x = +-0 the quotient may differ from IEEE in the sign of zero only; CollideBox returns
+// max(0, ...) and compares with <, so that sign never changes its result.
+__device__ __forceinline__ bool fastDirComponent(float d) {
+  const float a = fabsf(d);
+  return a >= 0x1p-30f && a <= 0x1p30f;
+}
+__device__ __forceinline__ bool fastCoord(float v) {
+  const float a = fabsf(v);
+  return a == 0.0f || (a >= 0x1p-37f && a <= 0x1p59f);
+}
+__device__ __forceinline__ float refinedRcp(float d) {
+  const float r0 = __builtin_amdgcn_rcpf(d);
+  const float e = __builtin_fmaf(-d, r0, 1.0f);
+  return __builtin_fmaf(e, r0, r0);
+}
+__device__ __forceinline__ float quotient(float x, float d, float r) {
+  float q = x * r;
+  float t = __builtin_fmaf(-d, q, x);
+  q = __builtin_fmaf(t, r, q);
+  t = __builtin_fmaf(-d, q, x);
+  return __builtin_fmaf(t, r, q);
+}
+
+struct RayRcp {
+  float rx, ry, rz;
+};
+
+// CollideBox with quotient(); no NaN can arise (finite x, non-zero d, no overflow), so min/max of
+// the slab values use the hardware min/max, which agree with the GLSL selects on non-NaN inputs up
+// to the sign of a zero result, which the final max(0, .) and the < comparison ignore.
+__device__ __forceinline__ float boxHitFast(const Ray3& r, const RayRcp& q, float mnx, float mny, float mnz,
+                                            float mxx, float mxy, float mxz, float tmax) {
+  const float t0x = quotient(mnx - r.ox, r.dx, q.rx), t0y = quotient(mny - r.oy, r.dy, q.ry),
+              t0z = quotient(mnz - r.oz, r.dz, q.rz);
+  const float t1x = quotient(mxx - r.ox, r.dx, q.rx), t1y = quotient(mxy - r.oy, r.dy, q.ry),
+              t1z = quotient(mxz - r.oz, r.dz, q.rz);
+  const float tmin = fmaxf(0.0f, fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z))));
+  const float tm = fminf(tmax, fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z))));
+  return (tmin < tm) ? tmin : -1.0f;
+}
+
 // CollideTriangle (collision.glsl:58-95) with edges precomputed; u, v written when accepted.
 __device__ __forceinline__ float triHit(const Ray3& r, const float4 t0, const float4 t1, const float4 t2, float tmax,
                                         float& uo, float& vo) {
@@ -88,9 +138,11 @@ struct Closest {
 
 // Walk the tree for one ray. ANY: stop at the first accepted triangle (returns true).
 // `stk` points at this lane's column of the block's LDS stack (stride kTraceBlock).
-template <bool ANY, bool COUNT, int STACK>
+template <bool ANY, bool COUNT, int STACK, bool FAST>
 __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const Ray3& r, float tmax, Closest& hit,
                                          Counters& cnt) {
+  RayRcp rq{};
+  if (FAST) rq = RayRcp{refinedRcp(r.dx), refinedRcp(r.dy), refinedRcp(r.dz)};
   uint32_t node = s.root;
   int sp = 0;
   if (!ANY) hit.t = tmax;
@@ -101,8 +153,14 @@ __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const
       const uint4 d = reinterpret_cast<const uint4*>(np)[3];
       if (COUNT) cnt.inner++;
       const float lim = ANY ? tmax : hit.t;
-      const float d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, lim);
-      const float d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+      float d0, d1;
+      if (FAST) {
+        d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+        d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+      } else {
+        d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+        d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+      }
       const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
       if (h0 && h1) {
         const bool n1 = d1 < d0;
@@ -142,6 +200,20 @@ __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const
   return false;
 }
 
+// Whether this ray may use boxHitFast (see quotient()); the scene flag covers the box bounds.
+__device__ __forceinline__ bool fastLane(const DevScene& s, const Ray3& r, float tmax) {
+  return s.fastBoxes && fastDirComponent(r.dx) && fastDirComponent(r.dy) && fastDirComponent(r.dz) &&
+         fastCoord(r.ox) && fastCoord(r.oy) && fastCoord(r.oz) && tmax == tmax;
+}
+
+// One traversal routine per wave: the fast slab division when every active lane qualifies.
+template <bool ANY, bool COUNT, int STACK>
+__device__ __forceinline__ bool traverseAuto(const DevScene& s, uint32_t* stk, const Ray3& r, float tmax,
+                                             Closest& hit, Counters& cnt) {
+  if (__all(fastLane(s, r, tmax) ? 1 : 0)) return traverse<ANY, COUNT, STACK, true>(s, stk, r, tmax, hit, cnt);
+  return traverse<ANY, COUNT, STACK, false>(s, stk, r, tmax, hit, cnt);
+}
+
 __device__ __forceinline__ Ray3 loadRay(const float4* rays, uint32_t i, float& tmax) {
   const float4 a = rays[2 * i], b = rays[2 * i + 1];
   tmax = a.w;
@@ -159,7 +231,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s, const f
   const Ray3 r = loadRay(rays, i, tmax);
   Closest h{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
   Counters c;
-  traverse<false, COUNT, STACK>(s, lds + threadIdx.x, r, tmax, h, c);
+  traverseAuto<false, COUNT, STACK>(s, lds + threadIdx.x, r, tmax, h, c);
   hits[i] = make_float4(__uint_as_float(h.tri), h.t, h.u, h.v);
   if (leafNid) leafNid[i] = h.nid;
   if (COUNT) {
@@ -179,13 +251,65 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s, const float
   const Ray3 r = loadRay(rays, i, tmax);
   Closest h{};
   Counters c;
-  const bool occ = traverse<true, COUNT, STACK>(s, lds + threadIdx.x, r, tmax, h, c);
+  const bool occ = traverseAuto<true, COUNT, STACK>(s, lds + threadIdx.x, r, tmax, h, c);
   occluded[i] = occ ? 1u : 0u;
   if (COUNT) {
     counters[3 * i] = c.inner;
     counters[3 * i + 1] = c.leaves;
     counters[3 * i + 2] = c.tests;
   }
+}
+
+// ---- camera rays generated in the kernel --------------------------------------------------
+struct Frustum4 {
+  float cam[3], c00[3], c10[3], c01[3];
+};
+
+// cameraPass.comp:50-54: stepW = (c10 - c00) * invW, stepH = (c01 - c00) * invH,
+// target = c00 + stepW * gx + stepH * gy, dir = normalize(target - cam), with
+// normalize(v) = v / sqrt(dot(v, v)) and invW = 1.f / W as fillPassResources computes it
+// (raytracingPass.cpp:47).
+__device__ __forceinline__ Ray3 cameraRay(const Frustum4& f, uint32_t W, uint32_t H, uint32_t gx, uint32_t gy) {
+  const float invW = 1.0f / float(W), invH = 1.0f / float(H);
+  const float fx = float(gx), fy = float(gy);
+  float d[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float sw = (f.c10[k] - f.c00[k]) * invW;
+    const float sh = (f.c01[k] - f.c00[k]) * invH;
+    const float target = (f.c00[k] + sw * fx) + sh * fy;
+    d[k] = target - f.cam[k];
+  }
+  const float len = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+  return Ray3{f.cam[0], f.cam[1], f.cam[2], d[0] / len, d[1] / len, d[2] / len};
+}
+
+// One block covers a 16x8 pixel region, each wave an 8x8 tile: neighbouring lanes trace
+// neighbouring pixels. referenceRows = 1 reproduces cameraPass.comp:46's storage row
+// 16*ceil(H/16) - gy (the generating row differs from the stored one and row 0 is never written).
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void kTraceCamera(DevScene s, Frustum4 f, uint32_t W, uint32_t H,
+                                                            uint32_t referenceRows, float4* __restrict__ hits,
+                                                            uint32_t* __restrict__ leafNid) {
+  __shared__ uint32_t lds[STACK * kTraceBlock];
+  const uint32_t tilesX = (W + 15u) / 16u;
+  const uint32_t b = xcdBlock();
+  const uint32_t bx = b % tilesX, by = b / tilesX;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t x = bx * 16u + wave * 8u + (lane & 7u), y = by * 8u + (lane >> 3);
+  if (x >= W || y >= H) return;
+  uint32_t gy = y;
+  if (referenceRows) {
+    if (y == 0) return;
+    gy = 16u * ((H + 15u) / 16u) - y;
+  }
+  const Ray3 r = cameraRay(f, W, H, x, gy);
+  Closest h{kTMaxF, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
+  Counters c;
+  traverseAuto<false, false, STACK>(s, lds + threadIdx.x, r, kTMaxF, h, c);
+  const size_t i = size_t(y) * W + x;
+  hits[i] = make_float4(__uint_as_float(h.tri), h.t, h.u, h.v);
+  if (leafNid) leafNid[i] = h.nid;
 }
 
 // computeShadowMask (bvhLighting.glsl:143-163) for one closest hit per lane.
@@ -211,7 +335,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowMasks(DevScene s, const fl
     const float px = r.ox + r.dx * t, py = r.oy + r.dy * t, pz = r.oz + r.dz * t;
     if (flags & 1u) {
       const Ray3 sr{px, py, pz, -sunX, -sunY, -sunZ};
-      if (traverse<true, false, STACK>(s, stk, sr, kTMaxF, dummy, c)) mask |= 1u;
+      if (traverseAuto<true, false, STACK>(s, stk, sr, kTMaxF, dummy, c)) mask |= 1u;
     }
     const uint32_t node = nid & 0x3FFFFFFFu;
     const uint32_t off = node < s.numNodes ? s.leafOff[node] : 0xFFFFFFFFu;
@@ -231,7 +355,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowMasks(DevScene s, const fl
         bool occ = true;
         if (len < L.w) {
           const Ray3 lr{px, py, pz, dx / len, dy / len, dz / len};
-          occ = traverse<true, false, STACK>(s, stk, lr, len, dummy, c);
+          occ = traverseAuto<true, false, STACK>(s, stk, lr, len, dummy, c);
         }
         if (occ) mask |= 1u << (li + 1u);
       }
@@ -240,11 +364,40 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowMasks(DevScene s, const fl
   masks[i] = mask;
 }
 
-// ---- tile light culling ---------------------------------------------------------------------
-struct Frustum4 {
-  float cam[3], c00[3], c10[3], c01[3];
-};
+// ---- self-test of quotient() against IEEE division --------------------------------------------
+__device__ __forceinline__ uint32_t mix32(uint64_t& s) {
+  s = s * 6364136223846793005ull + 1442695040888963407ull;
+  uint64_t z = s;
+  z = (z ^ (z >> 33)) * 0xff51afd7ed558ccdull;
+  z = (z ^ (z >> 33)) * 0xc4ceb9fe1a85ec53ull;
+  return uint32_t(z >> 32);
+}
 
+// Random (x, d) in the range where quotient() claims equality with x / d: |d| in [2^-30, 2^30),
+// |x| in [2^-60, 2^60), random signs and mantissas, a quarter of the mantissas forced to the
+// extremes (all ones / all zeros) where rounding ties are nearest. Counts bitwise differences.
+__global__ __launch_bounds__(256) void kDivSelfTest(uint64_t seed, uint32_t perThread,
+                                                    unsigned long long* __restrict__ mismatches) {
+  const uint64_t gid = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+  uint64_t st = seed ^ (gid * 0x9E3779B97F4A7C15ull);
+  unsigned long long bad = 0;
+  for (uint32_t k = 0; k < perThread; ++k) {
+    const uint32_t a = mix32(st), b = mix32(st), c = mix32(st);
+    uint32_t md = a & 0x7FFFFFu, mx = b & 0x7FFFFFu;
+    if ((c & 3u) == 0) md = (c & 4u) ? 0x7FFFFFu : 0u;
+    if ((c & 24u) == 0) mx = (c & 32u) ? 0x7FFFFFu : 0u;
+    const uint32_t ed = 127u - 30u + ((c >> 8) % 60u);
+    const uint32_t ex = 127u - 60u + ((c >> 16) % 120u);
+    const float d = __uint_as_float(((a >> 31) << 31) | (ed << 23) | md);
+    const float x = __uint_as_float(((b >> 31) << 31) | (ex << 23) | mx);
+    const float ieee = x / d;
+    const float fast = quotient(x, d, refinedRcp(d));
+    bad += (__float_as_uint(ieee) != __float_as_uint(fast)) ? 1ull : 0ull;
+  }
+  atomicAdd(mismatches, bad);
+}
+
+// ---- tile light culling ---------------------------------------------------------------------
 // computePlan (cullFrustum.glsl:9-13): n = normalize(cross(p1 - p2, p3 - p2)), w = -dot(n, p2).
 __device__ __forceinline__ void computePlane(const float* p1, const float* p2, const float* p3, float* out) {
   const float ax = p1[0] - p2[0], ay = p1[1] - p2[1], az = p1[2] - p2[2];

@@ -46,6 +46,7 @@ struct DevScene {
   uint32_t numNodes;         // reference node count
   uint32_t leafWords;
   uint32_t numLights;
+  uint32_t fastBoxes;        // 1 if every inner-node box bound qualifies for the fast slab division
 };
 
 }  // namespace rtq

@@ -272,6 +272,17 @@ rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, 
     d.d[2] = id;
     d.d[3] = 0;
   }
+  // the fast slab division needs every box bound to be 0 or of magnitude in [2^-37, 2^59]
+  bool fastBoxes = true;
+  for (const rtq::DevNode& d : dnodes) {
+    for (int k = 0; k < 4; ++k) {
+      const float v[3] = {d.a[k], d.b[k], d.c[k]};
+      for (float x : v) {
+        const float ax = std::fabs(x);
+        if (!(ax == 0.0f || (ax >= 0x1p-37f && ax <= 0x1p59f))) fastBoxes = false;
+      }
+    }
+  }
   std::vector<uint32_t> leafOff(numNodes);
   for (uint32_t id = 0; id < numNodes; ++id) leafOff[id] = nodes[16 * id + 3];
   std::vector<uint32_t> leafData(leaf, leaf + leafWords);
@@ -296,6 +307,7 @@ rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, 
   s->view.numNodes = numNodes;
   s->view.leafWords = leafWords;
   s->view.numLights = numLights;
+  s->view.fastBoxes = fastBoxes ? 1u : 0u;
   s->info.num_nodes = numNodes;
   s->info.num_inner = uint32_t(innerIds.size());
   s->info.num_leaf_slots = uint32_t(dtris.size());
@@ -364,6 +376,30 @@ rtq_status rtq_dispatch_rays_counted(rtq_scene* scene, const rtq_ray* d_rays, ui
                   d_counters, stream);
 }
 
+rtq_status rtq_trace_camera(rtq_scene* s, const rtq_frustum* f, uint32_t width, uint32_t height, uint32_t flags,
+                            rtq_hit* d_hits, uint32_t* d_leaf_nid, void* stream) {
+  if (!s || !f) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene or frustum");
+  if (width == 0 || height == 0) return RTQ_OK;
+  if (!d_hits) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL hit buffer");
+  if (flags & ~RTQ_CAMERA_REFERENCE_ROWS) return fail(RTQ_ERR_INVALID_ARGUMENT, "bad flags");
+  if (width > (1u << 16) || height > (1u << 16)) return fail(RTQ_ERR_LIMIT, "frame larger than 65536 pixels a side");
+  RTQ_HIP(hipSetDevice(s->dev->ordinal));
+  rtq::Frustum4 fr;
+  std::memcpy(&fr, f, sizeof(fr));
+  const uint32_t blocks = ((width + 15u) / 16u) * ((height + 7u) / 8u);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  float4* h = reinterpret_cast<float4*>(d_hits);
+  const uint32_t ref = flags & RTQ_CAMERA_REFERENCE_ROWS;
+  if (stackClass(s->info.max_depth) == 64)
+    hipLaunchKernelGGL((rtq::kTraceCamera<64>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, fr, width,
+                       height, ref, h, d_leaf_nid);
+  else
+    hipLaunchKernelGGL((rtq::kTraceCamera<32>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, fr, width,
+                       height, ref, h, d_leaf_nid);
+  RTQ_HIP(hipGetLastError());
+  return RTQ_OK;
+}
+
 rtq_status rtq_shadow_masks(rtq_scene* s, const rtq_ray* d_rays, const rtq_hit* d_hits, const uint32_t* d_leaf_nid,
                             uint32_t n, const float sun_dir[3], uint32_t flags, uint32_t* d_masks, void* stream) {
   if (!s) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene");
@@ -384,6 +420,27 @@ rtq_status rtq_shadow_masks(rtq_scene* s, const rtq_ray* d_rays, const rtq_hit* 
     hipLaunchKernelGGL((rtq::kShadowMasks<32>), grid, block, 0, st, s->view, r, h, d_leaf_nid, n, sd[0], sd[1], sd[2],
                        flags, d_masks);
   RTQ_HIP(hipGetLastError());
+  return RTQ_OK;
+}
+
+rtq_status rtq_selftest_division(rtq_device* dev, uint64_t seed, uint64_t count, uint64_t* mismatches) {
+  if (!dev || !mismatches) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  const uint32_t blocks = 4096, threads = 256;
+  const uint64_t per = (count + uint64_t(blocks) * threads - 1) / (uint64_t(blocks) * threads);
+  if (per > 0xFFFFFFFFull) return fail(RTQ_ERR_LIMIT, "count too large");
+  unsigned long long* d = nullptr;
+  RTQ_HIP(hipMalloc(&d, sizeof(unsigned long long)));
+  hipError_t e = hipMemset(d, 0, sizeof(unsigned long long));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(rtq::kDivSelfTest, dim3(blocks), dim3(threads), 0, nullptr, seed, uint32_t(per), d);
+    e = hipGetLastError();
+  }
+  unsigned long long h = 0;
+  if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) return hipFail(e, "division self-test");
+  *mismatches = h;
   return RTQ_OK;
 }
 

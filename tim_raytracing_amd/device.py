@@ -130,6 +130,20 @@ class Scene:
                    nid=d_n.read(np.uint32, n), hits=hits)
         return out
 
+    def trace_camera(self, frustum: B.Frustum, width: int, height: int, flags: int = 0):
+        n = width * height
+        d_h = self.dev.alloc(16 * n)
+        d_n = self.dev.alloc(4 * n)
+        zeros = np.zeros(n, B.HIT_DTYPE)
+        zeros["tri_id"] = 0xFFFFFFFF
+        d_h.write(zeros)
+        B.check_device("rtq_trace_camera",
+                       self.dev.lib.rtq_trace_camera(self.handle, C.byref(frustum), width, height, flags, d_h.ptr,
+                                                     d_n.ptr, None))
+        hits = d_h.read(B.HIT_DTYPE, n)
+        return {"tri_id": hits["tri_id"].copy(), "t": hits["t"].copy(), "u": hits["u"].copy(),
+                "v": hits["v"].copy(), "nid": d_n.read(np.uint32, n), "hits": hits}
+
     def trace_any(self, rays: np.ndarray, counted: bool = False):
         n = rays.size
         d_r = self.dev.upload(rays)

@@ -168,6 +168,86 @@ rtq_status rtq_cull_lights(rtq_scene* scene, const rtq_frustum* frustum, uint32_
                            uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity, uint32_t* h_total,
                            void* stream);
 
+/* ---- batched form: many scenes and ray sets in one persistent launch ----
+ * No counterpart in the reference (it dispatches one pass at a time, raytracingPass.cpp:89-110).
+ * Items reference resident scenes by index; each item's rays, outputs and t limits are exactly as in
+ * rtq_dispatch_rays, and results are identical to dispatching each item alone. */
+typedef struct rtq_batch rtq_batch;
+
+typedef struct rtq_batch_item {
+  uint32_t scene;    /* index into the scene array given to rtq_batch_create */
+  uint32_t flags;    /* RTQ_TRACE_CLOSEST or RTQ_TRACE_ANY */
+  uint32_t num_rays;
+  uint32_t reserved; /* 0 */
+  const rtq_ray* d_rays;
+  void* d_out;            /* rtq_hit[num_rays] or uint32_t[num_rays] */
+  uint32_t* d_leaf_nid;   /* closest hit only, may be NULL */
+} rtq_batch_item;
+
+typedef struct rtq_batch_params {
+  uint32_t chunk_rays;    /* rays a wave takes per grab: 64, 128, 256 or 512 */
+  uint32_t blocks_per_cu; /* persistent 128-lane blocks per compute unit, 1..16 */
+  uint32_t stack_depth;   /* per-lane LDS stack entries: 24, 32 or 64, >= the deepest scene */
+  uint32_t use_graph;     /* 1: replay a captured hipGraph (needs a non-NULL stream) */
+} rtq_batch_params;
+
+rtq_status rtq_batch_create(rtq_device* dev, rtq_scene* const* scenes, uint32_t num_scenes,
+                            const rtq_batch_item* items, uint32_t num_items, rtq_batch** out);
+rtq_status rtq_batch_destroy(rtq_batch* batch);
+rtq_status rtq_batch_set_params(rtq_batch* batch, const rtq_batch_params* params);
+rtq_status rtq_batch_get_params(const rtq_batch* batch, rtq_batch_params* params);
+/* One persistent launch over every item (asynchronous on `stream`). */
+rtq_status rtq_batch_launch(rtq_batch* batch, void* stream);
+/* The same items as one plain rtq_dispatch_rays launch each (for comparison). */
+rtq_status rtq_batch_launch_plain(rtq_batch* batch, void* stream);
+/* Times every valid (chunk_rays, blocks_per_cu, stack_depth) on this batch, `reps` launches each,
+ * keeps the fastest in the batch and returns it with its mean time. Synchronises `stream`. */
+rtq_status rtq_batch_autotune(rtq_batch* batch, uint32_t reps, rtq_batch_params* best, float* best_ms,
+                              void* stream);
+
+/* ---- served form: resident scenes, budgeted admission, continuous batching ----
+ * No counterpart in the reference. Scenes stay resident; requests (host rays in, host results
+ * out) are admitted while their device buffers fit the HBM budget, batched with whatever else is
+ * admitted, and traced by the persistent batch kernel; up to two batches run at once so new work
+ * starts while the previous batch drains. Traversal arithmetic is the same as rtq_dispatch_rays, so
+ * each result is identical whether the request runs alone or in a mix. rtq_server_submit and
+ * rtq_request_* are thread-safe. */
+typedef struct rtq_server rtq_server;
+typedef struct rtq_request rtq_request;
+
+typedef struct rtq_server_config {
+  uint64_t hbm_budget_bytes; /* device bytes for in-flight request buffers (52 B per ray) */
+  uint32_t max_queue;        /* waiting requests beyond this make submit return RTQ_ERR_QUEUE_FULL */
+  uint32_t max_batch_rays;   /* 0: no cap besides the budget */
+  uint32_t plain;            /* 1: one launch per request, no batching (comparison mode) */
+  uint32_t reserved;
+  rtq_batch_params params;   /* persistent-kernel settings, e.g. from rtq_batch_autotune */
+} rtq_server_config;
+
+typedef struct rtq_request_stats {
+  double queued_ms;   /* submit -> admitted */
+  double latency_ms;  /* submit -> results in host memory */
+  uint32_t batch_id;  /* launch that carried the request */
+  uint32_t batch_requests;
+} rtq_request_stats;
+
+rtq_status rtq_server_create(rtq_device* dev, rtq_scene* const* scenes, uint32_t num_scenes,
+                             const rtq_server_config* cfg, rtq_server** out);
+/* Waits for all admitted work, fails queued requests with RTQ_ERR_CANCELLED, frees the server. */
+rtq_status rtq_server_destroy(rtq_server* srv);
+/* host_rays, host_out (rtq_hit or uint32_t per ray) and host_leaf_nid (closest hit, may be NULL)
+ * must stay valid until the request completes. RTQ_ERR_BUDGET: the request alone exceeds the budget. */
+rtq_status rtq_server_submit(rtq_server* srv, uint32_t scene, uint32_t flags, const rtq_ray* host_rays,
+                             uint32_t num_rays, void* host_out, uint32_t* host_leaf_nid, rtq_request** req);
+/* Blocks up to timeout_ms (0xFFFFFFFF = forever); returns the request's final status, or
+ * RTQ_ERR_TIMEOUT if it is still running. */
+rtq_status rtq_request_wait(rtq_request* req, uint32_t timeout_ms);
+/* Cancels a request that has not been admitted yet (RTQ_ERR_INVALID_ARGUMENT otherwise). */
+rtq_status rtq_request_cancel(rtq_request* req);
+rtq_status rtq_request_get_stats(const rtq_request* req, rtq_request_stats* stats);
+/* Frees a completed request handle. */
+rtq_status rtq_request_release(rtq_request* req);
+
 /* Diagnostic: compares the traversal's reciprocal-based slab division with IEEE division on
  * `count` random operand pairs from the range where the kernels use it; *mismatches receives the
  * number of bitwise differences (0 expected). No counterpart in the reference. Synchronous. */

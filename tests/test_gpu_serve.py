@@ -1,0 +1,124 @@
+"""Served form: results identical alone and in the mix, admission under the HBM budget,
+queue-full backpressure and cancellation."""
+import threading
+
+import numpy as np
+import pytest
+
+from tests import gpu_util as G
+from tim_raytracing_amd import bindings as B
+from tim_raytracing_amd import device as D
+from tim_raytracing_amd import host as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes(n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        tris = int(np.exp(rng.uniform(np.log(2000), np.log(60000))))
+        b = H.BVHBuilder()
+        b.generate(B.RTQ_SCENE_SPHERES, 2000 + k, max(1, tris // 256))
+        b.build(16)
+        out.append(b)
+    return out
+
+
+def _requests(builders, count, seed, lo=200, hi=80000):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(count):
+        si = int(rng.integers(0, len(builders)))
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        rays = builders[si].surface_rays(seed=10 + k, count=n)
+        flags = B.RTQ_TRACE_CLOSEST if rng.random() < 0.5 else B.RTQ_TRACE_ANY
+        reqs.append((si, flags, rays))
+    return reqs
+
+
+def test_serve_mix_equals_alone():
+    print("case: 8 resident scenes, 40 requests from 4 client threads, results vs alone", flush=True)
+    dev = G.dev()
+    builders = _scenes(8, 7)
+    scenes = [D.Scene.from_builder(dev, b) for b in builders]
+    reqs = _requests(builders, 40, 8)
+    # alone: one dispatch per request
+    alone = []
+    for si, flags, rays in reqs:
+        if flags == B.RTQ_TRACE_CLOSEST:
+            g = scenes[si].trace_closest(rays)
+            alone.append((g["hits"].view(np.uint32).copy(), g["nid"]))
+        else:
+            alone.append((scenes[si].trace_any(rays), None))
+    # mixed: a budget that holds only part of the stream at once, 4 submitting threads
+    srv = D.Server(dev, scenes, budget_bytes=8 << 20)
+    handles = [None] * len(reqs)
+
+    def client(t):
+        for k in range(t, len(reqs), 4):
+            si, flags, rays = reqs[k]
+            handles[k] = srv.submit(si, flags, rays)
+
+    th = [threading.Thread(target=client, args=(t,)) for t in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    batches = set()
+    for k, h in enumerate(handles):
+        assert h.wait() == B.RTQ_OK
+        st = h.stats()
+        batches.add(st.batch_id)
+        assert st.latency_ms >= st.queued_ms >= 0
+        a_out, a_nid = alone[k]
+        if reqs[k][1] == B.RTQ_TRACE_CLOSEST:
+            assert np.array_equal(h.out.view(np.uint32), a_out)
+            assert np.array_equal(h.nid, a_nid)
+        else:
+            assert np.array_equal(h.out, a_out)
+        h.release()
+    assert len(batches) < len(reqs)  # requests were batched together
+    srv.close()
+    # plain comparison mode gives the same results
+    srv = D.Server(dev, scenes, budget_bytes=8 << 20, plain=True)
+    hs = [srv.submit(si, flags, rays) for si, flags, rays in reqs[:10]]
+    for k, h in enumerate(hs):
+        assert h.wait() == B.RTQ_OK
+        if reqs[k][1] == B.RTQ_TRACE_CLOSEST:
+            assert np.array_equal(h.out.view(np.uint32), alone[k][0])
+        else:
+            assert np.array_equal(h.out, alone[k][0])
+        h.release()
+    srv.close()
+
+
+def test_serve_admission_limits():
+    print("case: server budget rejection, queue-full, cancellation", flush=True)
+    dev = G.dev()
+    b = _scenes(1, 9)[0]
+    sc = D.Scene.from_builder(dev, b)
+    srv = D.Server(dev, [sc], budget_bytes=1 << 20, max_queue=3)
+    big = b.surface_rays(seed=1, count=40000)        # 52 B/ray > 1 MiB budget
+    with pytest.raises(B.RtqError) as e:
+        srv.submit(0, B.RTQ_TRACE_CLOSEST, big)
+    assert e.value.status == 7  # RTQ_ERR_BUDGET
+    small = b.surface_rays(seed=2, count=15000)       # ~780 KB: one fits at a time
+    hs = []
+    full = False
+    for k in range(40):
+        try:
+            hs.append(srv.submit(0, B.RTQ_TRACE_CLOSEST, small))
+        except B.RtqError as err:
+            assert err.status == 8  # RTQ_ERR_QUEUE_FULL
+            full = True
+            break
+    assert full
+    # cancel whatever is still queued; the rest completes
+    cancelled = sum(1 for h in hs if h.cancel() == B.RTQ_OK)
+    for h in hs:
+        st = h.wait()
+        assert st in (B.RTQ_OK, 9)
+        h.release()
+    assert cancelled >= 1
+    srv.close()

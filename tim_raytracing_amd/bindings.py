@@ -54,6 +54,26 @@ class BuildStats(C.Structure):
                 ("num_triangles", C.c_uint32), ("build_seconds", C.c_double)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("flags", C.c_uint32), ("num_rays", C.c_uint32), ("reserved", C.c_uint32),
+                ("d_rays", C.c_void_p), ("d_out", C.c_void_p), ("d_leaf_nid", C.c_void_p)]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("chunk_rays", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("stack_depth", C.c_uint32),
+                ("use_graph", C.c_uint32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_queue", C.c_uint32), ("max_batch_rays", C.c_uint32),
+                ("plain", C.c_uint32), ("reserved", C.c_uint32), ("params", BatchParams)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("queued_ms", C.c_double), ("latency_ms", C.c_double), ("batch_id", C.c_uint32),
+                ("batch_requests", C.c_uint32)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("num_nodes", C.c_uint32), ("num_inner", C.c_uint32), ("num_leaf_slots", C.c_uint32),
                 ("num_triangles", C.c_uint32), ("num_lights", C.c_uint32), ("max_depth", C.c_uint32),
@@ -111,6 +131,20 @@ _DEVICE_SIGS = {
     "rtq_shadow_masks": ([P, P, P, P, U32, P, U32, P, P], C.c_int),
     "rtq_cull_lights": ([P, C.POINTER(Frustum), U32, U32, P, P, U32, C.POINTER(U32), P], C.c_int),
     "rtq_selftest_division": ([P, U64, U64, C.POINTER(U64)], C.c_int),
+    "rtq_batch_create": ([P, P, U32, P, U32, C.POINTER(P)], C.c_int),
+    "rtq_batch_destroy": ([P], C.c_int),
+    "rtq_batch_set_params": ([P, C.POINTER(BatchParams)], C.c_int),
+    "rtq_batch_get_params": ([P, C.POINTER(BatchParams)], C.c_int),
+    "rtq_batch_launch": ([P, P], C.c_int),
+    "rtq_batch_launch_plain": ([P, P], C.c_int),
+    "rtq_batch_autotune": ([P, U32, C.POINTER(BatchParams), C.POINTER(C.c_float), P], C.c_int),
+    "rtq_server_create": ([P, P, U32, C.POINTER(ServerConfig), C.POINTER(P)], C.c_int),
+    "rtq_server_destroy": ([P], C.c_int),
+    "rtq_server_submit": ([P, U32, U32, P, U32, P, P, C.POINTER(P)], C.c_int),
+    "rtq_request_wait": ([P, U32], C.c_int),
+    "rtq_request_cancel": ([P], C.c_int),
+    "rtq_request_get_stats": ([P, C.POINTER(RequestStats)], C.c_int),
+    "rtq_request_release": ([P], C.c_int),
 }
 
 

@@ -364,6 +364,66 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowMasks(DevScene s, const fl
   masks[i] = mask;
 }
 
+// ---- batched form: many scenes and ray sets in one persistent launch --------------------------
+// Work is split into chunks of `chunkRays` consecutive rays of one item; items are sorted by
+// chunkBegin. Each wave takes the next chunk from a device counter until all chunks are gone,
+// so every wave reaches the exit test and the grid drains.
+struct BatchItemDev {
+  uint32_t scene, flags, numRays, chunkBegin;
+  const float4* rays;
+  void* out;
+  uint32_t* nid;
+  uint64_t pad;
+};
+static_assert(sizeof(BatchItemDev) == 48, "BatchItemDev is 48 B");
+
+__global__ void kResetCounter(uint32_t* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* __restrict__ scenes,
+                                                                const BatchItemDev* __restrict__ items,
+                                                                uint32_t numItems, uint32_t totalChunks,
+                                                                uint32_t chunkRays, uint32_t* __restrict__ counter) {
+  __shared__ uint32_t lds[STACK * kTraceBlock];
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t* stk = lds + threadIdx.x;
+  for (;;) {
+    uint32_t chunk = 0;
+    if (lane == 0) chunk = atomicAdd(counter, 1u);
+    chunk = __builtin_amdgcn_readfirstlane(__shfl(chunk, 0, 64));
+    if (chunk >= totalChunks) break;
+    uint32_t lo = 0, hi = numItems;
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (items[mid].chunkBegin <= chunk) lo = mid;
+      else hi = mid;
+    }
+    const BatchItemDev it = items[lo];
+    const DevScene s = scenes[it.scene];
+    const uint32_t base = (chunk - it.chunkBegin) * chunkRays;
+    for (uint32_t g = 0; g < chunkRays; g += 64u) {
+      const uint32_t i = base + g + lane;
+      if (i < it.numRays) {
+        float tmax;
+        const Ray3 r = loadRay(it.rays, i, tmax);
+        Counters c;
+        if (it.flags == 1u) {
+          Closest h{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
+          traverseAuto<false, false, STACK>(s, stk, r, tmax, h, c);
+          reinterpret_cast<float4*>(it.out)[i] = make_float4(__uint_as_float(h.tri), h.t, h.u, h.v);
+          if (it.nid) it.nid[i] = h.nid;
+        } else {
+          Closest h{};
+          const bool occ = traverseAuto<true, false, STACK>(s, stk, r, tmax, h, c);
+          reinterpret_cast<uint32_t*>(it.out)[i] = occ ? 1u : 0u;
+        }
+      }
+    }
+  }
+}
+
 // ---- self-test of quotient() against IEEE division --------------------------------------------
 __device__ __forceinline__ uint32_t mix32(uint64_t& s) {
   s = s * 6364136223846793005ull + 1442695040888963407ull;

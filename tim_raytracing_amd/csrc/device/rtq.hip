@@ -477,3 +477,6 @@ rtq_status rtq_cull_lights(rtq_scene* s, const rtq_frustum* f, uint32_t width, u
 }
 
 }  // extern "C"
+
+#include "batch.inc"
+#include "serve.inc"

@@ -69,6 +69,124 @@ class Device:
             self.handle = None
 
 
+class Batch:
+    """Batched form: many (scene, ray set) items in one persistent launch (rtq_batch_*)."""
+
+    def __init__(self, dev: Device, scenes, items):
+        """items: list of (scene_index, flags, d_rays_ptr, num_rays, d_out_ptr, d_nid_ptr_or_None)"""
+        self.dev = dev
+        arr = (C.c_void_p * max(1, len(scenes)))(*[s.handle.value for s in scenes])
+        its = (B.BatchItem * max(1, len(items)))()
+        for k, (si, flags, d_rays, n, d_out, d_nid) in enumerate(items):
+            its[k] = B.BatchItem(si, flags, n, 0, d_rays, d_out, d_nid)
+        h = C.c_void_p()
+        B.check_device("rtq_batch_create", dev.lib.rtq_batch_create(dev.handle, arr, len(scenes), its, len(items),
+                                                                    C.byref(h)))
+        self.handle = h
+        self._scenes = list(scenes)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            self.dev.lib.rtq_batch_destroy(h)
+            self.handle = None
+
+    def params(self) -> B.BatchParams:
+        p = B.BatchParams()
+        B.check_device("rtq_batch_get_params", self.dev.lib.rtq_batch_get_params(self.handle, C.byref(p)))
+        return p
+
+    def set_params(self, chunk_rays=None, blocks_per_cu=None, stack_depth=None, use_graph=None):
+        p = self.params()
+        if chunk_rays is not None:
+            p.chunk_rays = chunk_rays
+        if blocks_per_cu is not None:
+            p.blocks_per_cu = blocks_per_cu
+        if stack_depth is not None:
+            p.stack_depth = stack_depth
+        if use_graph is not None:
+            p.use_graph = int(use_graph)
+        B.check_device("rtq_batch_set_params", self.dev.lib.rtq_batch_set_params(self.handle, C.byref(p)))
+
+    def launch(self, stream=None):
+        B.check_device("rtq_batch_launch", self.dev.lib.rtq_batch_launch(self.handle, stream))
+
+    def launch_plain(self, stream=None):
+        B.check_device("rtq_batch_launch_plain", self.dev.lib.rtq_batch_launch_plain(self.handle, stream))
+
+    def autotune(self, reps=2, stream=None):
+        p = B.BatchParams()
+        ms = C.c_float()
+        B.check_device("rtq_batch_autotune", self.dev.lib.rtq_batch_autotune(self.handle, reps, C.byref(p),
+                                                                             C.byref(ms), stream))
+        return p, ms.value
+
+
+class Request:
+    def __init__(self, server: "Server", handle, rays, out, nid):
+        self.server = server
+        self.handle = handle
+        self._keep = (rays, out, nid)  # host buffers must outlive the request
+        self.out = out
+        self.nid = nid
+
+    def wait(self, timeout_ms: int = 0xFFFFFFFF) -> int:
+        return self.server.dev.lib.rtq_request_wait(self.handle, timeout_ms)
+
+    def cancel(self) -> int:
+        return self.server.dev.lib.rtq_request_cancel(self.handle)
+
+    def stats(self) -> B.RequestStats:
+        st = B.RequestStats()
+        B.check_device("rtq_request_get_stats", self.server.dev.lib.rtq_request_get_stats(self.handle, C.byref(st)))
+        return st
+
+    def release(self):
+        if self.handle:
+            B.check_device("rtq_request_release", self.server.dev.lib.rtq_request_release(self.handle))
+            self.handle = None
+
+
+class Server:
+    """Served form (rtq_server_*): resident scenes, HBM-budget admission, continuous batching."""
+
+    def __init__(self, dev: Device, scenes, budget_bytes: int, max_queue: int = 0, max_batch_rays: int = 0,
+                 plain: bool = False, params: B.BatchParams | None = None):
+        self.dev = dev
+        arr = (C.c_void_p * max(1, len(scenes)))(*[s.handle.value for s in scenes])
+        cfg = B.ServerConfig(budget_bytes, max_queue, max_batch_rays, int(plain), 0, params or B.BatchParams())
+        h = C.c_void_p()
+        B.check_device("rtq_server_create", dev.lib.rtq_server_create(dev.handle, arr, len(scenes), C.byref(cfg),
+                                                                      C.byref(h)))
+        self.handle = h
+        self._scenes = list(scenes)
+
+    def submit(self, scene: int, flags: int, rays: np.ndarray, out: np.ndarray | None = None,
+               nid: np.ndarray | None = None) -> Request:
+        n = rays.size
+        rays = np.ascontiguousarray(rays)
+        if out is None:
+            out = np.zeros(n, B.HIT_DTYPE) if flags == B.RTQ_TRACE_CLOSEST else np.zeros(n, np.uint32)
+        if nid is None and flags == B.RTQ_TRACE_CLOSEST:
+            nid = np.zeros(n, np.uint32)
+        h = C.c_void_p()
+        B.check_device("rtq_server_submit",
+                       self.dev.lib.rtq_server_submit(self.handle, scene, flags, rays.ctypes.data, n, out.ctypes.data,
+                                                      nid.ctypes.data if nid is not None else None, C.byref(h)))
+        return Request(self, h, rays, out, nid)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.dev.lib.rtq_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Scene:
     def __init__(self, dev: Device, blob: np.ndarray, ranges, positions: np.ndarray):
         self.dev = dev

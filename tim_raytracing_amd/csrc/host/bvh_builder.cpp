@@ -174,8 +174,8 @@ void BvhBuilder::addSphereLight(const SphereLightDesc& l) { lights_.push_back(l)
 // ---------------------------------------------------------------------------------------------
 // Split classification: fillSplitData<FillItems> (BVHBuilder.cpp:952-1094).
 template <bool Fill>
-void BvhBuilder::classify(const Ctx&, Split& out, const Box3& parent, const Box3& leftBox, const Box3& rightBox,
-                          const std::vector<u32>& items) const {
+bool BvhBuilder::classify(const Ctx&, Split& out, const Box3& parent, const Box3& leftBox, const Box3& rightBox,
+                          const std::vector<u32>& items, long bound) const {
   out.leftBox = leftBox;
   out.rightBox = rightBox;
   const float leftVolume = boxVolume(leftBox);
@@ -184,8 +184,16 @@ void BvhBuilder::classify(const Ctx&, Split& out, const Box3& parent, const Box3
   Box3 curLeft = leftBox, curRight = rightBox;
   const float thr = params_.expandNodeVolumeThreshold;
   const float meanSize = meanTriangleSize_;
+  const long n = long(items.size());
+  long k = 0;
 
   for (u32 id : items) {
+    if (!Fill && bound >= 0 && (k & 7) == 0) {
+      // each remaining item adds at most one to uniqueL or uniqueR
+      const long lo = long(out.uniqueL < out.uniqueR ? out.uniqueL : out.uniqueR);
+      if (lo + (n - k) <= bound) return false;
+    }
+    ++k;
     const bool inL = triangleBoxOverlap(v0_[id], v1_[id], v2_[id], leftBox);
     const bool inR = triangleBoxOverlap(v0_[id], v1_[id], v2_[id], rightBox);
     const Box3& aabb = aabb_[id];
@@ -265,6 +273,7 @@ void BvhBuilder::classify(const Ctx&, Split& out, const Box3& parent, const Box3
       }
     }
   }
+  return true;
 }
 
 // Count-only classification. The counts are sums of per-item decisions that do not depend on
@@ -307,8 +316,8 @@ static float splitScore(u32 uL, u32 uR) {
 }
 
 // searchBestSplit (BVHBuilder.cpp:1096-1183).
-void BvhBuilder::searchSplit(const Ctx& ctx, const TreeNode* node, const std::vector<u32>& items, int axis,
-                             Split& out) const {
+bool BvhBuilder::searchSplit(const Ctx& ctx, const TreeNode* node, const std::vector<u32>& items, int axis,
+                             Split& out, long need) const {
   const u32 n = u32(items.size());
   const Box3& ext = node->extent;
   const F3 boxDim = sub(ext.mx, ext.mn);
@@ -332,20 +341,28 @@ void BvhBuilder::searchSplit(const Ctx& ctx, const TreeNode* node, const std::ve
       else
         lo = newCut;
     }
+    return true;
   } else {
+    // The reference keeps the first candidate whose score (min of the unique counts) strictly
+    // beats all earlier ones. A candidate is skipped as soon as its count so far plus the items
+    // left cannot beat the current best or reach `need` (below `need` the axis loses the pick
+    // anyway); the kept candidate, and hence the tree, is the same.
     const F3 fixedStep = dropAxis(boxDim, axis);
-    float score = -1;
+    long best = -1;
+    bool anyValid = false, recorded = false;
     auto processStep = [&](F3 step) {
       Box3 leftBox = {ext.mn, add(add(ext.mn, fixedStep), step)};
       Box3 rightBox = {add(ext.mn, step), ext.mx};
-      if (checkBox(leftBox) && checkBox(rightBox)) {
-        Split s;
-        classify<false>(ctx, s, ext, leftBox, rightBox, items);
-        float cur = splitScore(s.uniqueL, s.uniqueR);
-        if (cur > score) {
-          score = cur;
-          out = std::move(s);
-        }
+      if (!(checkBox(leftBox) && checkBox(rightBox))) return;
+      anyValid = true;
+      const long bound = best > need - 1 ? best : need - 1;
+      Split s;
+      if (!classify<false>(ctx, s, ext, leftBox, rightBox, items, bound)) return;
+      const long cur = long(s.uniqueL < s.uniqueR ? s.uniqueL : s.uniqueR);
+      if (cur > bound) {
+        best = cur;
+        out = std::move(s);
+        recorded = true;
       }
     };
     processStep(keepAxis(mulS(boxDim, 0.5f), axis));
@@ -357,6 +374,8 @@ void BvhBuilder::searchSplit(const Ctx& ctx, const TreeNode* node, const std::ve
       processStep(keepAxis(sub(lo3, ext.mn), axis));
       processStep(keepAxis(sub(hi3, ext.mn), axis));
     }
+    // no valid candidate: `out` stays default-constructed, as in the reference
+    return recorded || !anyValid;
   }
 }
 
@@ -398,14 +417,18 @@ void BvhBuilder::buildRec(Ctx& ctx, u32 depth, u32 numUnique, std::vector<u32>& 
 
   Split best;
   if (n < 1024) {
+    // getBestSplit(getBestSplit(0, 1), 2) with getBestSplit(a, b) = score(a) > score(b) ? a : b
+    // (BVHBuilder.cpp:795-802): a later axis wins ties, so it only matters if it reaches the
+    // current winner's score, which is what `need` passes down.
     Split s[3];
-    for (int a = 0; a < 3; ++a) searchSplit(ctx, node, items, a, s[a]);
-    auto pick = [&](int a, int b) {
-      return splitScore(s[a].uniqueL, s[a].uniqueR) > splitScore(s[b].uniqueL, s[b].uniqueR) ? a : b;
-    };
-    best = std::move(s[pick(pick(0, 1), 2)]);
+    auto score = [&](int a) { return long(s[a].uniqueL < s[a].uniqueR ? s[a].uniqueL : s[a].uniqueR); };
+    searchSplit(ctx, node, items, 0, s[0], 0);
+    int cur = 0;
+    for (int a = 1; a < 3; ++a)
+      if (searchSplit(ctx, node, items, a, s[a], score(cur)) && !(score(cur) > score(a))) cur = a;
+    best = std::move(s[cur]);
   } else {
-    searchSplit(ctx, node, items, int(depth % 3), best);
+    searchSplit(ctx, node, items, int(depth % 3), best, 0);
   }
 
   if (depth > 0) {

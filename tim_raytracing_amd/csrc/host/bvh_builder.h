@@ -91,10 +91,15 @@ class BvhBuilder {
   struct Ctx;
   void buildRec(Ctx& ctx, u32 depth, u32 numUnique, std::vector<u32>& items, TreeNode* node, bool parallel);
   void fillLeaf(TreeNode* node, const std::vector<u32>* items);
-  void searchSplit(const Ctx& ctx, const TreeNode* node, const std::vector<u32>& items, int axis, Split& out) const;
+  // Returns false when every valid candidate of this axis scores below `need` (the axis then cannot
+  // win the cross-axis pick and `out` is meaningless); otherwise `out` is the reference's choice.
+  bool searchSplit(const Ctx& ctx, const TreeNode* node, const std::vector<u32>& items, int axis, Split& out,
+                   long need) const;
+  // Fill = false with bound >= 0: stops (returns false) as soon as min(uniqueL, uniqueR) can no
+  // longer exceed `bound` whatever the remaining items do.
   template <bool Fill>
-  void classify(const Ctx& ctx, Split& out, const Box3& parent, const Box3& left, const Box3& right,
-                const std::vector<u32>& items) const;
+  bool classify(const Ctx& ctx, Split& out, const Box3& parent, const Box3& left, const Box3& right,
+                const std::vector<u32>& items, long bound = -1) const;
   void countSplit(const Ctx& ctx, Split& out, const Box3& parent, const Box3& left, const Box3& right,
                   const std::vector<u32>& items) const;
   void flatten(std::unique_ptr<TreeNode> root);

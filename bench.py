@@ -40,6 +40,15 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads", type=int, default=0, help="host threads for build and CPU baseline")
+    ap.add_argument("--mode", choices=("primary", "incoherent", "batch", "serve"), default="primary",
+                    help="primary: the headline workload; incoherent: 3 Mi-tri soup, 2 diffuse bounces + tile "
+                         "culling; batch: many scenes in one persistent launch vs one launch each; serve: "
+                         "Poisson request stream, continuous batching vs one launch per request")
+    ap.add_argument("--scale", type=float, default=1.0, help="triangle-count scale for incoherent/batch/serve")
+    ap.add_argument("--distinct", type=int, default=0,
+                    help="batch/serve: number of distinct BVHs built; scene slots reuse them round-robin "
+                         "(0 = one build per slot)")
+    ap.add_argument("--requests", type=int, default=240, help="serve: requests in the stream")
     return ap.parse_args()
 
 
@@ -84,8 +93,12 @@ def main():
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of "
                                    "this build", "n_gpus": args.gpus}))
         sys.exit(2)
-
-    import torch
+    if args.mode == "batch":
+        return bench_batch(args)
+    if args.mode == "serve":
+        return bench_serve(args)
+    if args.mode == "incoherent":
+        return bench_incoherent(args)
 
     from tim_raytracing_amd import bindings as B
     from tim_raytracing_amd import device as D
@@ -101,61 +114,49 @@ def main():
     n = rays.size
     setup_s = time.perf_counter() - t0
 
-    torch.cuda.init()
     dev = D.Device(0)
     scene = D.Scene(dev, blob, ranges, pos)
     info = scene.info()
-    stream = torch.cuda.current_stream()
-    sptr = stream.cuda_stream
-    d_rays = torch.from_numpy(rays.view(np.uint8).copy()).cuda()
-    d_hits = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
-    d_nid = torch.empty(n, dtype=torch.int32, device="cuda")
-    d_masks = torch.empty(n, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
+    stream = dev.stream()
+    sptr = stream.ptr
+    d_rays = dev.upload(rays)
+    d_hits = dev.alloc(16 * n)
+    d_nid = dev.alloc(4 * n)
+    d_masks = dev.alloc(4 * n)
+    ev0, ev1 = dev.event(), dev.event()
 
     def closest():
-        scene.dispatch(d_rays.data_ptr(), n, B.RTQ_TRACE_CLOSEST, d_hits.data_ptr(), d_nid.data_ptr(), sptr)
+        scene.dispatch(d_rays.ptr, n, B.RTQ_TRACE_CLOSEST, d_hits.ptr, d_nid.ptr, sptr)
+
+    def timed(fn, warmup, steps):
+        for _ in range(warmup):
+            fn()
+        stream.synchronize()
+        ev0.record(sptr)
+        for _ in range(steps):
+            fn()
+        ev1.record(sptr)
+        return ev0.elapsed_ms(ev1) / steps
 
     # per-ray visit counts for the algorithmic bytes (counting variant, untimed)
-    d_cnt = torch.empty(n * 3, dtype=torch.int32, device="cuda")
-    scene.dispatch_counted(d_rays.data_ptr(), n, B.RTQ_TRACE_CLOSEST, d_hits.data_ptr(), d_nid.data_ptr(),
-                           d_cnt.data_ptr(), sptr)
-    torch.cuda.synchronize()
-    cnt = d_cnt.cpu().numpy().astype(np.int64).reshape(n, 3)
+    d_cnt = dev.alloc(12 * n)
+    scene.dispatch_counted(d_rays.ptr, n, B.RTQ_TRACE_CLOSEST, d_hits.ptr, d_nid.ptr, d_cnt.ptr, sptr)
+    stream.synchronize()
+    cnt = d_cnt.read(np.uint32, 3 * n).astype(np.int64).reshape(n, 3)
     sum_inner, sum_leaf, sum_tri = (int(x) for x in cnt.sum(0))
     alg_bytes = 48 * n + 64 * sum_inner + 4 * sum_leaf + 48 * sum_tri
-    del d_cnt
+    d_cnt.free()
 
-    for _ in range(args.warmup):
-        closest()
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        closest()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / args.steps
-
-    hits = d_hits.cpu().numpy().view(B.HIT_DTYPE)
+    ms = timed(closest, args.warmup, args.steps)
+    hits = d_hits.read(B.HIT_DTYPE, n)
     hit_mask = hits["tri_id"] != B.RTQ_NO_HIT
     n_hit = int(hit_mask.sum())
 
     # shadow pass: one any-hit ray per hit toward the point light (light 0 of every leaf list)
     zero_sun = np.zeros(3, np.float32)
-    for _ in range(max(1, args.warmup // 2)):
-        scene.shadow_masks(d_rays.data_ptr(), d_hits.data_ptr(), d_nid.data_ptr(), n, zero_sun, 0,
-                           d_masks.data_ptr(), sptr)
-    ev0.record(stream)
-    for _ in range(args.steps):
-        scene.shadow_masks(d_rays.data_ptr(), d_hits.data_ptr(), d_nid.data_ptr(), n, zero_sun, 0,
-                           d_masks.data_ptr(), sptr)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    shadow_ms = ev0.elapsed_time(ev1) / args.steps
-    masks = d_masks.cpu().numpy()
+    shadow_ms = timed(lambda: scene.shadow_masks(d_rays.ptr, d_hits.ptr, d_nid.ptr, n, zero_sun, 0, d_masks.ptr, sptr),
+                      max(1, args.warmup // 2), args.steps)
+    masks = d_masks.read(np.uint32, n)
 
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     result = {
@@ -207,6 +208,238 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(blob, ranges, pos, rays, threads, args.cpu_seconds)
     print(json.dumps(result), flush=True)
+
+
+def _loguniform(rng, lo, hi, n):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+
+
+def _scene_pool(sizes, distinct, threads):
+    """Build one sphere-soup BVH per distinct size (UV spheres of 256 triangles, seeds 1000+i)."""
+    from tim_raytracing_amd import bindings as B
+    from tim_raytracing_amd import host as H
+
+    builders = []
+    for i, tris in enumerate(sizes[:distinct]):
+        b = H.BVHBuilder()
+        b.generate(B.RTQ_SCENE_SPHERES, 1000 + i, max(1, int(round(tris / 256))))
+        b.build(threads)
+        builders.append(b)
+    return builders
+
+
+def _primary_rays(n, seed):
+    from tim_raytracing_amd import host as H
+
+    side = int(np.ceil(np.sqrt(n)))
+    f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 1.0)
+    return np.ascontiguousarray(H.frustum_rays(f, side, side, seed=seed)[:n])
+
+
+def bench_batch(args):
+    """Batched form: 256 scenes of LogU[1K, 2M] triangles, LogU[4K, 1M] rays each (half primary,
+    half bounce), one persistent launch (autotuned, hipGraph replay) vs one launch per item."""
+    from tim_raytracing_amd import bindings as B
+    from tim_raytracing_amd import device as D
+
+    threads = host_threads(args.threads)
+    rng = np.random.default_rng(4)
+    slots = 256
+    sizes = _loguniform(rng, 1000, 2e6 * args.scale, slots)
+    distinct = args.distinct or slots
+    t0 = time.perf_counter()
+    builders = _scene_pool(sizes, distinct, threads)
+    build_s = time.perf_counter() - t0
+    dev = D.Device(0)
+    scenes = [D.Scene.from_builder(dev, b) for b in builders]
+    ray_counts = _loguniform(rng, 4096, 1 << 20, slots).astype(np.int64)
+    items, keep = [], []
+    total = 0
+    for i in range(slots):
+        si = i % distinct
+        n = int(ray_counts[i])
+        rays = _primary_rays(n, 10 + i) if i % 2 == 0 else builders[si].surface_rays(seed=10 + i, count=n)
+        d_r = dev.upload(rays)
+        d_o = dev.alloc(16 * n)
+        d_n = dev.alloc(4 * n)
+        keep.append((d_r, d_o, d_n))
+        items.append((si, B.RTQ_TRACE_CLOSEST, d_r.ptr, n, d_o.ptr, d_n.ptr))
+        total += n
+    batch = D.Batch(dev, scenes, items)
+    stream = dev.stream()
+    sptr = stream.ptr
+    ev0, ev1 = dev.event(), dev.event()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        stream.synchronize()
+        ev0.record(sptr)
+        for _ in range(args.steps):
+            fn()
+        ev1.record(sptr)
+        return ev0.elapsed_ms(ev1) / args.steps
+
+    plain_ms = timed(lambda: batch.launch_plain(sptr))
+    best, tuned_ms = batch.autotune(reps=2, stream=sptr)
+    batch.set_params(use_graph=1)
+    batched_ms = timed(lambda: batch.launch(sptr))
+    p = batch.params()
+    print(json.dumps({
+        "metric": "batched closest-hit Mrays/s vs one launch per item",
+        "value": round(total / (batched_ms * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(batched_ms, 4),
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch-256-scenes", "scene_slots": slots, "distinct_bvhs": distinct,
+                   "triangles_per_slot": "LogU[1e3, %g]" % (2e6 * args.scale),
+                   "distinct_triangles": int(sum(b.stats().num_triangles for b in builders)),
+                   "rays": int(total), "rays_per_item": "LogU[4096, 1048576]", "mix": "50% primary, 50% bounce"},
+        "plain_mrays_s": round(total / (plain_ms * 1e-3) / 1e6, 2), "plain_ms": round(plain_ms, 4),
+        "speedup_vs_plain": round(plain_ms / batched_ms, 3),
+        "autotuned": {"chunk_rays": p.chunk_rays, "blocks_per_cu": p.blocks_per_cu, "stack_depth": p.stack_depth,
+                      "use_graph": p.use_graph, "tuning_ms": round(tuned_ms, 4)},
+        "host": {"threads": threads, "build_s": round(build_s, 1)},
+    }), flush=True)
+
+
+def bench_serve(args):
+    """Served form: 64 resident scenes of LogU[10K, 3M] triangles, a Poisson stream of requests of
+    LogU[1K, 2M] rays (half closest, half any hit) at 1.5x the plain capacity; continuous batching vs
+    one launch per request under the same HBM budget."""
+    import threading
+
+    from tim_raytracing_amd import bindings as B
+    from tim_raytracing_amd import device as D
+
+    threads = host_threads(args.threads)
+    rng = np.random.default_rng(5)
+    slots = 64
+    sizes = _loguniform(rng, 1e4, 3e6 * args.scale, slots)
+    distinct = args.distinct or slots
+    t0 = time.perf_counter()
+    builders = _scene_pool(sizes, distinct, threads)
+    build_s = time.perf_counter() - t0
+    dev = D.Device(0)
+    scenes = [D.Scene.from_builder(dev, builders[i % distinct]) for i in range(slots)]
+    pool_rays = 1 << 21
+    pools = [builders[i].surface_rays(seed=50 + i, count=pool_rays) for i in range(distinct)]
+    nreq = args.requests
+    req_sizes = _loguniform(rng, 1000, 2e6, nreq).astype(np.int64)
+    req_scene = rng.integers(0, slots, nreq)
+    req_any = rng.random(nreq) < 0.5
+    req_off = [int(rng.integers(0, pool_rays - n + 1)) for n in req_sizes]
+    budget = 2 << 30
+
+    def run(plain, arrivals):
+        srv = D.Server(dev, scenes, budget_bytes=budget, plain=plain)
+        handles = [None] * nreq
+        start = time.perf_counter()
+
+        def client():
+            for k in range(nreq):
+                if arrivals is not None:
+                    wait = start + arrivals[k] - time.perf_counter()
+                    if wait > 0:
+                        time.sleep(wait)
+                si = int(req_scene[k])
+                rays = pools[si % distinct][req_off[k]:req_off[k] + int(req_sizes[k])]
+                flags = B.RTQ_TRACE_ANY if req_any[k] else B.RTQ_TRACE_CLOSEST
+                handles[k] = srv.submit(si, flags, rays)
+
+        th = threading.Thread(target=client)
+        th.start()
+        th.join()
+        lat = []
+        for h in handles:
+            assert h.wait() == B.RTQ_OK
+            lat.append(h.stats().latency_ms)
+        end = time.perf_counter()
+        batches = len({h.stats().batch_id for h in handles})
+        for h in handles:
+            h.release()
+        srv.close()
+        lat = np.array(lat)
+        return {"mrays_s": round(float(req_sizes.sum()) / (end - start) / 1e6, 2),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "makespan_s": round(end - start, 3), "launches": batches}
+
+    burst = run(True, None)  # plain capacity: everything submitted at once
+    capacity = burst["mrays_s"] * 1e6
+    rate = 1.5 * capacity / float(req_sizes.mean())
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    served = run(False, arrivals)
+    plain = run(True, arrivals)
+    print(json.dumps({
+        "metric": "served aggregate Mrays/s and p50/p99 latency vs one launch per request",
+        "value": served["mrays_s"], "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "serve-64-scenes-poisson", "scene_slots": slots, "distinct_bvhs": distinct,
+                   "triangles_per_slot": "LogU[1e4, %g]" % (3e6 * args.scale), "requests": nreq,
+                   "rays_per_request": "LogU[1000, 2e6]", "mix": "50% closest, 50% any",
+                   "arrival_rate_per_s": round(rate, 2), "offered_load": "1.5x plain capacity",
+                   "hbm_budget_bytes": budget},
+        "served": served, "plain": plain, "plain_burst_capacity": burst,
+        "speedup_vs_plain": round(served["mrays_s"] / plain["mrays_s"], 3) if plain["mrays_s"] else None,
+        "host": {"threads": threads, "build_s": round(build_s, 1)},
+    }), flush=True)
+
+
+def bench_incoherent(args):
+    """3 Mi-triangle Gaussian-mixture soup with 4096 point lights: 4 Mi diffuse bounce rays from
+    surfaces, a second bounce from their hits, and 16x16 tile culling of the lights at 1920x1080."""
+    from tim_raytracing_amd import bindings as B
+    from tim_raytracing_amd import device as D
+    from tim_raytracing_amd import host as H
+
+    threads = host_threads(args.threads)
+    t0 = time.perf_counter()
+    b, f, W, Hh = H.workload(3, scale=args.scale, threads=threads)
+    build_s = time.perf_counter() - t0
+    dev = D.Device(0)
+    scene = D.Scene.from_builder(dev, b)
+    n = 4 << 20
+    rays1 = b.surface_rays(seed=3, count=n)
+    stream = dev.stream()
+    sptr = stream.ptr
+    ev0, ev1 = dev.event(), dev.event()
+    d_h = dev.alloc(16 * n)
+    d_n = dev.alloc(4 * n)
+
+    def trace(rays):
+        d_r = dev.upload(rays)
+        fn = lambda: scene.dispatch(d_r.ptr, n, B.RTQ_TRACE_CLOSEST, d_h.ptr, d_n.ptr, sptr)
+        for _ in range(args.warmup):
+            fn()
+        stream.synchronize()
+        ev0.record(sptr)
+        for _ in range(args.steps):
+            fn()
+        ev1.record(sptr)
+        ms = ev0.elapsed_ms(ev1) / args.steps
+        return ms, d_h.read(B.HIT_DTYPE, n)
+
+    ms1, hits1 = trace(rays1)
+    rays2 = b.bounce_rays(4, rays1, hits1)
+    ms2, hits2 = trace(rays2)
+    ntiles = ((W + 15) // 16) * ((Hh + 15) // 16)
+    offs, lists = scene.cull_lights(f, W, Hh)
+    t = time.perf_counter()
+    reps = max(1, args.steps)
+    for _ in range(reps):
+        scene.cull_lights(f, W, Hh, capacity=int(lists.size))
+    cull_ms = (time.perf_counter() - t) * 1e3 / reps
+    print(json.dumps({
+        "metric": "incoherent closest-hit Mrays/s (2 diffuse bounces) + tile light culling",
+        "value": round(2 * n / ((ms1 + ms2) * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "soup-3M-bounce2-cull4096", "triangles": int(b.stats().num_triangles),
+                   "lights": 4096, "rays_per_bounce": n, "tiles": ntiles},
+        "bounce1_mrays_s": round(n / (ms1 * 1e-3) / 1e6, 2), "bounce2_mrays_s": round(n / (ms2 * 1e-3) / 1e6, 2),
+        "bounce1_hit_rate": round(float((hits1["tri_id"] != B.RTQ_NO_HIT).mean()), 4),
+        "bounce2_hit_rate": round(float((hits2["tri_id"] != B.RTQ_NO_HIT).mean()), 4),
+        "cull_ms_incl_host_sync": round(cull_ms, 3), "tile_light_pairs": int(lists.size),
+        "host": {"threads": threads, "build_s": round(build_s, 1)},
+    }), flush=True)
 
 
 if __name__ == "__main__":

@@ -110,6 +110,17 @@ rtq_status rtq_device_alloc(rtq_device* dev, size_t bytes, void** d_ptr);
 rtq_status rtq_device_free(rtq_device* dev, void* d_ptr);
 rtq_status rtq_copy_to_device(rtq_device* dev, void* d_dst, const void* h_src, size_t bytes, void* stream);
 
+/* Streams and events of the library's HIP runtime, for callers that need ordering and timing
+ * without a second runtime in the process. Streams have the default priority. */
+rtq_status rtq_stream_create(rtq_device* dev, void** stream);
+rtq_status rtq_stream_destroy(rtq_device* dev, void* stream);
+rtq_status rtq_stream_synchronize(rtq_device* dev, void* stream);
+rtq_status rtq_event_create(rtq_device* dev, void** event);
+rtq_status rtq_event_destroy(rtq_device* dev, void* event);
+rtq_status rtq_event_record(void* event, void* stream);
+/* Milliseconds between two recorded events; waits for `end`. */
+rtq_status rtq_event_elapsed_ms(void* start, void* end, float* ms);
+
 /* upload-BVH + upload-triangles. `blob`/`ranges` are exactly what BVHBuilder::fillGpuBuffer wrote
  * (BVHBuilder.cpp:1319-1534); `positions_xyz` holds the BVHGeometry vertex positions (12 B each).
  * The blob is converted once into the device layout and stays resident in HBM until

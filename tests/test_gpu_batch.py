@@ -82,8 +82,8 @@ def test_batch_matches_plain_and_oracle():
         else:
             occ, _ = sc.trace_any(rays, threads=16)
             assert np.array_equal(h, occ)
-    stream = __import__("torch").cuda.Stream()
-    sptr = stream.cuda_stream
+    stream = dev.stream()
+    sptr = stream.ptr
     for chunk, blocks, stack, graph in ((64, 2, 24, 0), (128, 4, 32, 0), (512, 8, 64, 0), (256, 6, 32, 1)):
         for (d_r, d_o, d_n) in bufs:
             d_o.write(np.zeros(d_o.nbytes, np.uint8))

@@ -121,6 +121,13 @@ _DEVICE_SIGS = {
     "rtq_device_alloc": ([P, C.c_size_t, C.POINTER(P)], C.c_int),
     "rtq_device_free": ([P, P], C.c_int),
     "rtq_copy_to_device": ([P, P, P, C.c_size_t, P], C.c_int),
+    "rtq_stream_create": ([P, C.POINTER(P)], C.c_int),
+    "rtq_stream_destroy": ([P, P], C.c_int),
+    "rtq_stream_synchronize": ([P, P], C.c_int),
+    "rtq_event_create": ([P, C.POINTER(P)], C.c_int),
+    "rtq_event_destroy": ([P, P], C.c_int),
+    "rtq_event_record": ([P, P], C.c_int),
+    "rtq_event_elapsed_ms": ([P, P, C.POINTER(C.c_float)], C.c_int),
     "rtq_scene_upload": ([P, P, C.c_size_t, C.POINTER(Ranges), P, U32, C.POINTER(P)], C.c_int),
     "rtq_scene_destroy": ([P], C.c_int),
     "rtq_scene_get_info": ([P, C.POINTER(SceneInfo)], C.c_int),

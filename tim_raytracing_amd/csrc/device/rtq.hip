@@ -145,6 +145,57 @@ rtq_status rtq_copy_to_device(rtq_device* dev, void* d_dst, const void* h_src, s
   return RTQ_OK;
 }
 
+rtq_status rtq_stream_create(rtq_device* dev, void** stream) {
+  if (!dev || !stream) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  hipStream_t s = nullptr;
+  RTQ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return RTQ_OK;
+}
+
+rtq_status rtq_stream_destroy(rtq_device* dev, void* stream) {
+  if (!dev) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL device");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  if (stream) RTQ_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return RTQ_OK;
+}
+
+rtq_status rtq_stream_synchronize(rtq_device* dev, void* stream) {
+  if (!dev) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL device");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  RTQ_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return RTQ_OK;
+}
+
+rtq_status rtq_event_create(rtq_device* dev, void** event) {
+  if (!dev || !event) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  hipEvent_t e = nullptr;
+  RTQ_HIP(hipEventCreate(&e));
+  *event = e;
+  return RTQ_OK;
+}
+
+rtq_status rtq_event_destroy(rtq_device* dev, void* event) {
+  if (!dev) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL device");
+  if (event) RTQ_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return RTQ_OK;
+}
+
+rtq_status rtq_event_record(void* event, void* stream) {
+  if (!event) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL event");
+  RTQ_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+  return RTQ_OK;
+}
+
+rtq_status rtq_event_elapsed_ms(void* start, void* end, float* ms) {
+  if (!start || !end || !ms) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  RTQ_HIP(hipEventSynchronize(static_cast<hipEvent_t>(end)));
+  RTQ_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)));
+  return RTQ_OK;
+}
+
 rtq_status rtq_read_hits(rtq_device* dev, const void* d_src, void* h_dst, size_t bytes, void* stream) {
   if (!dev || (!d_src && bytes) || (!h_dst && bytes)) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
   RTQ_HIP(hipSetDevice(dev->ordinal));

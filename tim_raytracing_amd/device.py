@@ -47,6 +47,51 @@ class DeviceBuffer:
         return out
 
 
+class Stream:
+    """A HIP stream of the library's runtime (default priority)."""
+
+    def __init__(self, dev: "Device"):
+        self.dev = dev
+        h = C.c_void_p()
+        B.check_device("rtq_stream_create", dev.lib.rtq_stream_create(dev.handle, C.byref(h)))
+        self.ptr = h.value
+
+    def synchronize(self):
+        B.check_device("rtq_stream_synchronize", self.dev.lib.rtq_stream_synchronize(self.dev.handle, self.ptr))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.dev.lib.rtq_stream_destroy(self.dev.handle, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self, dev: "Device"):
+        self.dev = dev
+        h = C.c_void_p()
+        B.check_device("rtq_event_create", dev.lib.rtq_event_create(dev.handle, C.byref(h)))
+        self.ptr = h.value
+
+    def record(self, stream=None):
+        B.check_device("rtq_event_record", self.dev.lib.rtq_event_record(self.ptr, stream))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        B.check_device("rtq_event_elapsed_ms", self.dev.lib.rtq_event_elapsed_ms(self.ptr, end.ptr, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.dev.lib.rtq_event_destroy(self.dev.handle, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
 class Device:
     def __init__(self, ordinal: int = 0):
         self.lib = B.device_lib()
@@ -56,6 +101,12 @@ class Device:
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
+
+    def stream(self) -> Stream:
+        return Stream(self)
+
+    def event(self) -> Event:
+        return Event(self)
 
     def upload(self, arr: np.ndarray) -> DeviceBuffer:
         buf = DeviceBuffer(self, np.asarray(arr).nbytes)

@@ -121,6 +121,10 @@ rtq_status rtq_event_record(void* event, void* stream);
 /* Milliseconds between two recorded events; waits for `end`. */
 rtq_status rtq_event_elapsed_ms(void* start, void* end, float* ms);
 
+/* Device-to-device copy with the library's own streaming kernel (16-byte loads and stores); also
+ * the reference point for HBM counter calibration. `bytes` must be a multiple of 16. */
+rtq_status rtq_device_copy(rtq_device* dev, void* d_dst, const void* d_src, size_t bytes, void* stream);
+
 /* upload-BVH + upload-triangles. `blob`/`ranges` are exactly what BVHBuilder::fillGpuBuffer wrote
  * (BVHBuilder.cpp:1319-1534); `positions_xyz` holds the BVHGeometry vertex positions (12 B each).
  * The blob is converted once into the device layout and stays resident in HBM until

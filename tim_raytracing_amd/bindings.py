@@ -121,6 +121,7 @@ _DEVICE_SIGS = {
     "rtq_device_alloc": ([P, C.c_size_t, C.POINTER(P)], C.c_int),
     "rtq_device_free": ([P, P], C.c_int),
     "rtq_copy_to_device": ([P, P, P, C.c_size_t, P], C.c_int),
+    "rtq_device_copy": ([P, P, P, C.c_size_t, P], C.c_int),
     "rtq_stream_create": ([P, C.POINTER(P)], C.c_int),
     "rtq_stream_destroy": ([P, P], C.c_int),
     "rtq_stream_synchronize": ([P, P], C.c_int),

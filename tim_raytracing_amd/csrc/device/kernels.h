@@ -424,6 +424,12 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* 
   }
 }
 
+// ---- streaming copy ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kCopy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+  const size_t stride = size_t(gridDim.x) * 256u;
+  for (size_t i = size_t(blockIdx.x) * 256u + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ---- self-test of quotient() against IEEE division --------------------------------------------
 __device__ __forceinline__ uint32_t mix32(uint64_t& s) {
   s = s * 6364136223846793005ull + 1442695040888963407ull;

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -77,7 +78,18 @@ void freeScene(rtq_scene* s) {
   delete s;
 }
 
-int stackClass(uint32_t depth) { return depth <= 32 ? 32 : 64; }
+// Smallest per-lane LDS stack (entries) that holds the deepest path of a scene: a smaller stack
+// leaves room for more resident waves per CU.
+int stackClass(uint32_t depth) { return depth <= 24 ? 24 : (depth <= 32 ? 32 : 64); }
+
+template <class F>
+void withStack(uint32_t depth, F&& f) {
+  switch (stackClass(depth)) {
+    case 24: f(std::integral_constant<int, 24>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    default: f(std::integral_constant<int, 64>{}); break;
+  }
+}
 
 }  // namespace
 
@@ -142,6 +154,18 @@ rtq_status rtq_copy_to_device(rtq_device* dev, void* d_dst, const void* h_src, s
   RTQ_HIP(hipSetDevice(dev->ordinal));
   RTQ_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)));
   RTQ_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return RTQ_OK;
+}
+
+rtq_status rtq_device_copy(rtq_device* dev, void* d_dst, const void* d_src, size_t bytes, void* stream) {
+  if (!dev || ((!d_dst || !d_src) && bytes)) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (bytes % 16 || reinterpret_cast<uintptr_t>(d_dst) % 16 || reinterpret_cast<uintptr_t>(d_src) % 16)
+    return fail(RTQ_ERR_INVALID_ARGUMENT, "copy needs 16-byte aligned pointers and size");
+  if (bytes == 0) return RTQ_OK;
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  hipLaunchKernelGGL(rtq::kCopy16, dim3(8192), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<uint4*>(d_dst), static_cast<const uint4*>(d_src), bytes / 16);
+  RTQ_HIP(hipGetLastError());
   return RTQ_OK;
 }
 
@@ -390,26 +414,18 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(rays);
-  const bool deep = stackClass(s->info.max_depth) == 64;
-  if (flags == RTQ_TRACE_CLOSEST) {
-    float4* h = static_cast<float4*>(out);
-    if (counters) {
-      if (deep) hipLaunchKernelGGL((rtq::kTraceClosest<64, true>), grid, block, 0, st, s->view, r, n, h, nid, counters);
-      else hipLaunchKernelGGL((rtq::kTraceClosest<32, true>), grid, block, 0, st, s->view, r, n, h, nid, counters);
+  withStack(s->info.max_depth, [&](auto S) {
+    constexpr int K = decltype(S)::value;
+    if (flags == RTQ_TRACE_CLOSEST) {
+      float4* h = static_cast<float4*>(out);
+      if (counters) hipLaunchKernelGGL((rtq::kTraceClosest<K, true>), grid, block, 0, st, s->view, r, n, h, nid, counters);
+      else hipLaunchKernelGGL((rtq::kTraceClosest<K, false>), grid, block, 0, st, s->view, r, n, h, nid, nullptr);
     } else {
-      if (deep) hipLaunchKernelGGL((rtq::kTraceClosest<64, false>), grid, block, 0, st, s->view, r, n, h, nid, nullptr);
-      else hipLaunchKernelGGL((rtq::kTraceClosest<32, false>), grid, block, 0, st, s->view, r, n, h, nid, nullptr);
+      uint32_t* o = static_cast<uint32_t*>(out);
+      if (counters) hipLaunchKernelGGL((rtq::kTraceAny<K, true>), grid, block, 0, st, s->view, r, n, o, counters);
+      else hipLaunchKernelGGL((rtq::kTraceAny<K, false>), grid, block, 0, st, s->view, r, n, o, nullptr);
     }
-  } else {
-    uint32_t* o = static_cast<uint32_t*>(out);
-    if (counters) {
-      if (deep) hipLaunchKernelGGL((rtq::kTraceAny<64, true>), grid, block, 0, st, s->view, r, n, o, counters);
-      else hipLaunchKernelGGL((rtq::kTraceAny<32, true>), grid, block, 0, st, s->view, r, n, o, counters);
-    } else {
-      if (deep) hipLaunchKernelGGL((rtq::kTraceAny<64, false>), grid, block, 0, st, s->view, r, n, o, nullptr);
-      else hipLaunchKernelGGL((rtq::kTraceAny<32, false>), grid, block, 0, st, s->view, r, n, o, nullptr);
-    }
-  }
+  });
   RTQ_HIP(hipGetLastError());
   return RTQ_OK;
 }
@@ -441,12 +457,11 @@ rtq_status rtq_trace_camera(rtq_scene* s, const rtq_frustum* f, uint32_t width, 
   hipStream_t st = static_cast<hipStream_t>(stream);
   float4* h = reinterpret_cast<float4*>(d_hits);
   const uint32_t ref = flags & RTQ_CAMERA_REFERENCE_ROWS;
-  if (stackClass(s->info.max_depth) == 64)
-    hipLaunchKernelGGL((rtq::kTraceCamera<64>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, fr, width,
+  withStack(s->info.max_depth, [&](auto S) {
+    constexpr int K = decltype(S)::value;
+    hipLaunchKernelGGL((rtq::kTraceCamera<K>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, fr, width,
                        height, ref, h, d_leaf_nid);
-  else
-    hipLaunchKernelGGL((rtq::kTraceCamera<32>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, fr, width,
-                       height, ref, h, d_leaf_nid);
+  });
   RTQ_HIP(hipGetLastError());
   return RTQ_OK;
 }
@@ -464,12 +479,11 @@ rtq_status rtq_shadow_masks(rtq_scene* s, const rtq_ray* d_rays, const rtq_hit* 
   const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(d_rays);
   const float4* h = reinterpret_cast<const float4*>(d_hits);
-  if (stackClass(s->info.max_depth) == 64)
-    hipLaunchKernelGGL((rtq::kShadowMasks<64>), grid, block, 0, st, s->view, r, h, d_leaf_nid, n, sd[0], sd[1], sd[2],
+  withStack(s->info.max_depth, [&](auto S) {
+    constexpr int K = decltype(S)::value;
+    hipLaunchKernelGGL((rtq::kShadowMasks<K>), grid, block, 0, st, s->view, r, h, d_leaf_nid, n, sd[0], sd[1], sd[2],
                        flags, d_masks);
-  else
-    hipLaunchKernelGGL((rtq::kShadowMasks<32>), grid, block, 0, st, s->view, r, h, d_leaf_nid, n, sd[0], sd[1], sd[2],
-                       flags, d_masks);
+  });
   RTQ_HIP(hipGetLastError());
   return RTQ_OK;
 }

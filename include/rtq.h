@@ -105,6 +105,15 @@ const char* rtq_last_error(void);
 rtq_status rtq_device_create(int hip_ordinal, rtq_device** out);
 rtq_status rtq_device_destroy(rtq_device* dev);
 
+/* Dispatch tuning. RTQ_OPTION_PERSISTENT: 1 = rtq_dispatch_rays uses a fixed grid of waves that
+ * refill finished lanes with new rays (results identical; default 1), 0 = one lane per ray over a
+ * grid of ceil(n/128) blocks. RTQ_OPTION_BLOCKS_PER_CU: persistent blocks of 128 lanes per compute
+ * unit (1..16, default 8). */
+#define RTQ_OPTION_PERSISTENT 1u
+#define RTQ_OPTION_BLOCKS_PER_CU 2u
+rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t value);
+rtq_status rtq_device_get_option(const rtq_device* dev, uint32_t option, uint32_t* value);
+
 /* Plain device-memory helpers for callers without their own allocator. */
 rtq_status rtq_device_alloc(rtq_device* dev, size_t bytes, void** d_ptr);
 rtq_status rtq_device_free(rtq_device* dev, void* d_ptr);
@@ -158,6 +167,23 @@ rtq_status rtq_dispatch_rays_counted(rtq_scene* scene, const rtq_ray* d_rays, ui
  * d_hits / d_leaf_nid hold width*height records, pixel (x, y) at y*width + x, y = 0 at corner00. */
 rtq_status rtq_trace_camera(rtq_scene* scene, const rtq_frustum* frustum, uint32_t width, uint32_t height,
                             uint32_t flags, rtq_hit* d_hits, uint32_t* d_leaf_nid, void* stream);
+
+/* Light-probe field placement (LightProbFieldHeader, Shaders/lightprob/lightprob.glsl:16-21, as the
+ * camera pass fills it from the scene box and field size, cameraPass.comp:62-66). */
+typedef struct rtq_lpf {
+  float aabb_min[3];
+  float aabb_max[3];
+  uint32_t resolution[3];
+} rtq_lpf;
+
+/* The camera pass's full TRACING_STEP output (cameraPass.comp:69-75), 8 uint32 per pixel at
+ * 2*(y*width + x): [vertexOffset, index01, index2_matId, floatBits(t)] of the closest hit and
+ * [shadowMask, probMask, 0, 0] (computeShadowMask with `sun_dir`/`shadow_flags` as in
+ * rtq_shadow_masks; computeProbMask when lpf is not NULL, else 0). A miss has t = 100, zero triangle
+ * words and zero masks. `flags` as for rtq_trace_camera. */
+rtq_status rtq_trace_camera_record(rtq_scene* scene, const rtq_frustum* frustum, uint32_t width, uint32_t height,
+                                   uint32_t flags, const float sun_dir[3], uint32_t shadow_flags, const rtq_lpf* lpf,
+                                   uint32_t* d_records, void* stream);
 
 /* read-hits: synchronising copy of `bytes` from device memory written by a dispatch on `stream`. */
 rtq_status rtq_read_hits(rtq_device* dev, const void* d_src, void* h_dst, size_t bytes, void* stream);

@@ -157,6 +157,7 @@ class RayTracingPass {
   ~RayTracingPass() {
     dev_.free(hits_);
     dev_.free(nids_);
+    if (records_) dev_.free(records_);
   }
   RayTracingPass(const RayTracingPass&) = delete;
   RayTracingPass& operator=(const RayTracingPass&) = delete;
@@ -179,11 +180,27 @@ class RayTracingPass {
   const rtq_hit* deviceHits() const { return hits_; }
   const uint32_t* deviceLeafNids() const { return nids_; }
 
+  // The camera pass with TRACING_STEP (cameraPass.comp:69-75): m_tracingResult's 2 x uvec4 per pixel
+  // (hit triangle words + floatBits(t), shadow mask + probe mask). lpf = NULL leaves the probe mask 0.
+  void traceRecord(const BVHData& bvh, const rtq_frustum& frustum, const float sunDir[3], uint32_t shadowFlags,
+                   const rtq_lpf* lpf, void* stream = nullptr, uint32_t flags = 0) {
+    if (!records_) records_ = static_cast<uint32_t*>(dev_.alloc(32 * size_t(w_) * h_));
+    check(rtq_trace_camera_record(bvh.scene(), &frustum, w_, h_, flags, sunDir, shadowFlags, lpf, records_, stream),
+          "rtq_trace_camera_record");
+  }
+  std::vector<uint32_t> readRecords(void* stream = nullptr) const {
+    std::vector<uint32_t> out(8 * size_t(w_) * h_);
+    if (records_) check(rtq_read_hits(dev_.handle(), records_, out.data(), out.size() * 4, stream), "rtq_read_hits");
+    return out;
+  }
+  const uint32_t* deviceRecords() const { return records_; }
+
  private:
   Device& dev_;
   uint32_t w_, h_;
   rtq_hit* hits_ = nullptr;
   uint32_t* nids_ = nullptr;
+  uint32_t* records_ = nullptr;
 };
 
 }  // namespace rtq

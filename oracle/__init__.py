@@ -12,6 +12,9 @@ Parity pins (what checks each oracle function, see tests/test_oracle_pins.py):
                         invariance under node renumbering; planted tie-break/shrink mistakes fail
   traverse (any)        == (closest raw t < tmax) on every ray; order independence
   shadow masks          scenes with a known occluder / no occluder
+  probe masks           closed forms: a probe field over an open scene (all 0), a wall between the
+                        hit and every probe corner (all 1), one cell split by a wall (the corner
+                        bits on the far side set, the near side clear)
   light culling         closed forms (light on the view axis, light behind the camera, huge radius)
 """
 from __future__ import annotations
@@ -41,6 +44,7 @@ _SIGS = {
     "orc_tile_planes": ([P, U32, U32, U32, U32, P], None),
     "orc_cull_lights": ([P, U32, P, U32, U32, P, P, C.c_uint64], C.c_uint64),
     "orc_camera_rays": ([P, U32, U32, C.c_int, P], None),
+    "orc_trace_record": ([P, C.c_size_t, P, P, U32, P, U32, U32, C.c_int, P, U32, P, P, P, C.c_int], C.c_int),
 }
 
 _cache = {}
@@ -139,6 +143,22 @@ class Scene:
                                     threads)
         assert rc == 0
         return masks
+
+    def trace_record(self, frustum, width: int, height: int, reference_rows: bool = False, sun_dir=None,
+                     shadow_flags: int = 0, lpf=None, threads: int = 8, plant: int = 0) -> np.ndarray:
+        """The camera pass's TRACING_STEP record, (H*W, 8) uint32; lpf = (min3, max3, res3) or None."""
+        fr = frustum_array(frustum)
+        rec = np.zeros((width * height, 8), np.uint32)
+        sd = np.asarray(sun_dir if sun_dir is not None else (0, 0, 0), np.float32)
+        box = res = None
+        if lpf is not None:
+            box = np.array(list(lpf[0]) + list(lpf[1]), np.float32)
+            res = np.array(lpf[2], np.uint32)
+        rc = lib(plant).orc_trace_record(_p(self.blob), self.blob.size, _p(self.ranges), _p(self.pos), self.nv, _p(fr),
+                                    width, height, int(reference_rows), _p(sd), shadow_flags, _p(box), _p(res),
+                                    _p(rec), threads)
+        assert rc == 0
+        return rec
 
     def cull_lights(self, frustum, width: int, height: int):
         lights = self.lights()

@@ -34,7 +34,8 @@ typedef uint32_t u32;
 
 #ifdef ORC_PLANT
 /* Planted mistakes, used only by the tests that check the pins catch them:
- * 1 = tie-break to the far child (d1 <= d0), 2 = no t shrink on acceptance, 3 = u clamp dropped. */
+ * 1 = tie-break to the far child (d1 <= d0), 2 = no t shrink on acceptance, 3 = u clamp dropped,
+ * 4 = probe cell without the half-cell shift, 5 = probe cell not capped at resolution - 2. */
 static const int kPlant = ORC_PLANT;
 #else
 static const int kPlant = 0;
@@ -531,6 +532,45 @@ static int any_hit(const orc_view* v, const float* o, const float* d, float tmax
   return traverse_bitstack(v, o, d, tmax, 1, &h);
 }
 
+/* computeShadowMask at hit position pos of the hit leaf nid. */
+static u32 shadow_mask_at(const orc_view* v, const float pos[3], u32 hit_nid, const float* sun, u32 flags) {
+  u32 mask = 0;
+  if (flags & 1u) {
+    float sd[3] = {-sun[0], -sun[1], -sun[2]};
+    mask |= any_hit(v, pos, sd, TMAX) ? 1u : 0u;
+  }
+  u32 nid = hit_nid & NID_MASK;
+  if (nid >= v->num_nodes) return mask;
+  u32 off = v->nodes[16 * nid + 3];
+  if (off != 0xFFFFFFFFu && off < v->leaf_words) {
+    u32 packed = v->leaf[off];
+    /* unpackLightCountAndOffset (:133-141) with NodeTriangleStride 3 */
+    u32 lo = 3u * (packed & 0xFFFu) + ((packed >> 12) & 0xFFu) + ((packed >> 20) & 0xFFu);
+    u32 count = (packed >> 28) & 0xFu;
+    lo += 1u + off;
+    for (u32 k = 0; k < count; ++k) {
+      if (lo + k >= v->leaf_words) break;
+      u32 li = v->leaf[lo + k];
+      if (li + 1u >= 32u || li >= v->num_lights) continue;
+      const float* L = v->lights + 14 * (size_t)li;
+      if (((const u32*)L)[0] != 1u) continue; /* Light_Sphere */
+      float lp[3] = {L[1], L[2], L[3]};
+      float radius = L[4];
+      float dir[3] = {lp[0] - pos[0], lp[1] - pos[1], lp[2] - pos[2]};
+      float len = sqrtf((dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2]);
+      int occ;
+      if (len < radius) {
+        float nd[3] = {dir[0] / len, dir[1] / len, dir[2] / len};
+        occ = any_hit(v, pos, nd, len);
+      } else {
+        occ = 1;
+      }
+      if (occ) mask |= 1u << (li + 1u);
+    }
+  }
+  return mask;
+}
+
 static void* run_shadow(void* arg) {
   orc_shadow_job* j = (orc_shadow_job*)arg;
   const orc_view* v = j->v;
@@ -540,37 +580,7 @@ static void* run_shadow(void* arg) {
     u32 mask = 0;
     if (t < TMAX && j->hit_nid[i] != 0xFFFFFFFFu) {
       float pos[3] = {r[0] + r[4] * t, r[1] + r[5] * t, r[2] + r[6] * t};
-      if (j->flags & 1u) {
-        float sd[3] = {-j->sun[0], -j->sun[1], -j->sun[2]};
-        mask |= any_hit(v, pos, sd, TMAX) ? 1u : 0u;
-      }
-      u32 nid = j->hit_nid[i] & NID_MASK;
-      u32 off = v->nodes[16 * nid + 3];
-      if (off != 0xFFFFFFFFu && off < v->leaf_words) {
-        u32 packed = v->leaf[off];
-        /* unpackLightCountAndOffset (:133-141) with NodeTriangleStride 3 */
-        u32 lo = 3u * (packed & 0xFFFu) + ((packed >> 12) & 0xFFu) + ((packed >> 20) & 0xFFu);
-        u32 count = (packed >> 28) & 0xFu;
-        lo += 1u + off;
-        for (u32 k = 0; k < count; ++k) {
-          u32 li = v->leaf[lo + k];
-          if (li + 1u >= 32u || li >= v->num_lights) continue;
-          const float* L = v->lights + 14 * (size_t)li;
-          if (((const u32*)L)[0] != 1u) continue; /* Light_Sphere */
-          float lp[3] = {L[1], L[2], L[3]};
-          float radius = L[4];
-          float dir[3] = {lp[0] - pos[0], lp[1] - pos[1], lp[2] - pos[2]};
-          float len = sqrtf((dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2]);
-          int occ;
-          if (len < radius) {
-            float nd[3] = {dir[0] / len, dir[1] / len, dir[2] / len};
-            occ = any_hit(v, pos, nd, len);
-          } else {
-            occ = 1;
-          }
-          if (occ) mask |= 1u << (li + 1u);
-        }
-      }
+      mask = shadow_mask_at(v, pos, j->hit_nid[i], j->sun, j->flags);
     }
     j->masks[i] = mask;
   }
@@ -632,6 +642,112 @@ void orc_camera_rays(const float* frustum, u32 width, u32 height, int reference_
       r[6] = d[2] / len;
       r[7] = 0.0f;
     }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * computeProbMask (Shaders/bvh/bvhLighting.glsl:165-184): the probe cell around pos from
+ * getLightProbCoordUVW (lightprob/lightprobHelpers.glsl:70-76) with step = computeLpfStep
+ * (:10-14), cell corner = min(int(uvw), resolution - 2), and 8 any-hit segments of t limit 1 toward
+ * getLightProbPosition (:62-68) of each corner, the direction left unnormalised (createShadowRay,
+ * bvhLighting.glsl:87-98). int() of NaN is undefined in GLSL; it is fixed to 0 here. */
+static u32 prob_mask_at(const orc_view* v, const float pos[3], const float* lpf_box, const u32* res) {
+  const float* mn = lpf_box;
+  const float* mx = lpf_box + 3;
+  u32 coord[3];
+  for (int k = 0; k < 3; ++k) {
+    float step = (mx[k] - mn[k]) / (float)res[k];
+    float uvw = ((pos[k] - mn[k]) / step) - (kPlant == 4 ? 0.0f : 0.5f);
+    uvw = gmin(gmax(uvw, 0.0f), (float)res[k] - 1.0f);
+    int c = (uvw == uvw) ? (int)uvw : 0;
+    u32 lim = kPlant == 5 ? res[k] - 1u : res[k] - 2u;
+    coord[k] = ((u32)c < lim) ? (u32)c : lim;
+  }
+  u32 mask = 0;
+  for (u32 i = 0; i < 8; ++i) {
+    u32 off[3] = {i & 1u, (i >> 1) & 1u, (i >> 2) & 1u};
+    float d[3];
+    for (int k = 0; k < 3; ++k) {
+      float step = (mx[k] - mn[k]) / (float)res[k];
+      float probe = mn[k] + step * ((float)(coord[k] + off[k]) + 0.5f);
+      d[k] = probe - pos[k];
+    }
+    if (any_hit(v, pos, d, 1.0f)) mask |= 1u << i;
+  }
+  return mask;
+}
+
+/* The camera pass's TRACING_STEP (Shaders/cameraPass.comp:69-75): per pixel 8 words,
+ * ClosestHit_getHitData (bvhBindings.glsl:85-92) = the hit leaf's inline triangle triple and
+ * floatBits(t), then computeAdditionalHitData (baseRaytracingPass.glsl:85-97) = shadow and probe
+ * masks when t < TMAX. A miss writes zero triangle words (the shader's are stale) and zero masks.
+ * lpf_box = (min.xyz, max.xyz) and lpf_res, or NULL for no probe mask. Row 0 is left untouched
+ * when reference_rows is set. */
+typedef struct {
+  const orc_view* v;
+  const float* rays;
+  u32 width, height;
+  int reference_rows;
+  const float* sun;
+  u32 shadow_flags;
+  const float* lpf_box;
+  const u32* lpf_res;
+  u32* records;
+  u32 begin, end;
+} orc_record_job;
+
+static void* run_record(void* arg) {
+  orc_record_job* j = (orc_record_job*)arg;
+  const orc_view* v = j->v;
+  for (u32 i = j->begin; i < j->end; ++i) {
+    if (j->reference_rows && i < j->width) continue;
+    const float* r = j->rays + 8 * (size_t)i;
+    orc_hit h;
+    memset(&h, 0, sizeof(h));
+    h.t = TMAX;
+    h.slot = 0xFFFFFFFFu;
+    h.nid = 0xFFFFFFFFu;
+    traverse_bitstack(v, r, r + 4, TMAX, 0, &h);
+    u32* out = j->records + 8 * (size_t)i;
+    memset(out, 0, 8 * sizeof(u32));
+    if (h.slot != 0xFFFFFFFFu) {
+      out[0] = v->leaf[h.slot];
+      out[1] = v->leaf[h.slot + 1];
+      out[2] = v->leaf[h.slot + 2];
+    }
+    memcpy(&out[3], &h.t, 4);
+    if (h.t < TMAX && h.nid != 0xFFFFFFFFu) {
+      float pos[3] = {r[0] + r[4] * h.t, r[1] + r[5] * h.t, r[2] + r[6] * h.t};
+      out[4] = shadow_mask_at(v, pos, h.nid, j->sun, j->shadow_flags);
+      if (j->lpf_box) out[5] = prob_mask_at(v, pos, j->lpf_box, j->lpf_res);
+    }
+  }
+  return 0;
+}
+
+int orc_trace_record(const uint8_t* blob, size_t bytes, const u32* ranges, const float* pos, u32 nv,
+                     const float* frustum, u32 width, u32 height, int reference_rows, const float* sun_dir,
+                     u32 shadow_flags, const float* lpf_box, const u32* lpf_res, u32* records, int threads) {
+  orc_view view;
+  if (view_init(&view, blob, bytes, ranges, pos, nv)) { view_free(&view); return -1; }
+  size_t n = (size_t)width * height;
+  float* rays = (float*)malloc(sizeof(float) * 8 * (n ? n : 1));
+  if (!rays) { view_free(&view); return -2; }
+  orc_camera_rays(frustum, width, height, reference_rows, rays);
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  pthread_t th[256];
+  orc_record_job jobs[256];
+  for (int t = 0; t < threads; ++t) {
+    orc_record_job jj = {&view, rays, width, height, reference_rows, sun_dir, shadow_flags, lpf_box, lpf_res,
+                         records, (u32)(n * t / threads), (u32)(n * (t + 1) / threads)};
+    jobs[t] = jj;
+  }
+  for (int t = 1; t < threads; ++t) pthread_create(&th[t], 0, run_record, &jobs[t]);
+  run_record(&jobs[0]);
+  for (int t = 1; t < threads; ++t) pthread_join(th[t], 0);
+  free(rays);
+  view_free(&view);
+  return 0;
 }
 
 /* ---------------------------------------------------------------------------------------------

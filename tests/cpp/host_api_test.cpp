@@ -12,6 +12,10 @@ int orc_trace_closest(const uint8_t* blob, size_t bytes, const uint32_t* ranges,
                       const float* rays, uint32_t n, uint32_t* tri_id, float* t, float* u, float* v, uint32_t* nid,
                       uint32_t* slot, uint32_t* counters, int form, int threads);
 void orc_camera_rays(const float* frustum, uint32_t width, uint32_t height, int reference_rows, float* out);
+int orc_trace_record(const uint8_t* blob, size_t bytes, const uint32_t* ranges, const float* pos, uint32_t nv,
+                     const float* frustum, uint32_t width, uint32_t height, int reference_rows, const float* sun_dir,
+                     uint32_t shadow_flags, const float* lpf_box, const uint32_t* lpf_res, uint32_t* records,
+                     int threads);
 }
 
 int main() {
@@ -65,6 +69,24 @@ int main() {
     }
     if (bad || hit < n / 4) {
       std::fprintf(stderr, "mismatches %zu, hits %zu of %zu\n", bad, hit, n);
+      return 1;
+    }
+    // the full TRACING_STEP record (reference row order, sun + probe masks)
+    const float sun[3] = {0.6f, 0.0f, -0.8f};
+    rtq_lpf lpf{{0, 0, -1}, {8, 8, 4}, {8, 8, 5}};
+    pass.traceRecord(data, f, sun, RTQ_SHADOW_SUN, &lpf, nullptr, RTQ_CAMERA_REFERENCE_ROWS);
+    std::vector<uint32_t> rec = pass.readRecords();
+    std::vector<uint32_t> orec(8 * n, 0);
+    const float box[6] = {0, 0, -1, 8, 8, 4};
+    orc_trace_record(blob.data(), blob.size(), rr, p.first, p.second, &f.camera_pos[0], 320, 240, 1, sun,
+                     RTQ_SHADOW_SUN, box, lpf.resolution, orec.data(), 4);
+    size_t recBad = 0, masked = 0;
+    for (size_t i = 320; i < n; ++i) {
+      recBad += std::memcmp(&rec[8 * i], &orec[8 * i], 32) != 0;
+      masked += orec[8 * i + 5] != 0;
+    }
+    if (recBad || masked < n / 8) {
+      std::fprintf(stderr, "record mismatches %zu, probe-masked pixels %zu\n", recBad, masked);
       return 1;
     }
     // a failing call throws with its status

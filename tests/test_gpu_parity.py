@@ -62,6 +62,66 @@ def test_camera_pass_rays_generated_in_kernel():
                 assert (g["tri_id"][:w] == B.RTQ_NO_HIT).all()  # row 0 is never written
 
 
+def test_camera_record_matches_oracle():
+    _print("case: camera-pass TRACING_STEP record (hit words, t, shadow and probe masks), 2 scenes, both row modes")
+    from tests.test_oracle_pins import _probe_scene
+    b = _probe_scene()
+    blob, r = b.fill_gpu_buffer()
+    probe = {"gpu": D.Scene(G.dev(), blob, r, b.positions()), "orc": oracle.Scene(blob, r, b.positions())}
+    s2 = np.float32(np.sqrt(np.float32(0.5)))
+    cases = [(probe, H.camera_frustum((2.3, 2.0, 5.0), (0, 0, -1), (0, 1, 0), H.REFERENCE_FOV, 1.0), 96, 96,
+              (-s2, 0, -s2), ((0, 0, -1), (4, 4, 3), (4, 4, 4)))]
+    sp = G.scene(2, 1 / 16)
+    mn, mx = sp["builder"].scene_box()
+    sun = np.array([0.3, -0.2, -0.93], np.float32)
+    sun /= np.linalg.norm(sun)
+    cases.append((sp, H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 333 / 200), 333, 200,
+                  tuple(sun), (tuple(mn), tuple(mx), (16, 8, 12))))
+    for sc, f, w, h, sd, lpf in cases:
+        for ref_rows in (0, 1):
+            for use_lpf in (True, False):
+                lp = lpf if use_lpf else None
+                g = sc["gpu"].trace_record(f, w, h, flags=ref_rows, sun_dir=sd, shadow_flags=B.RTQ_SHADOW_SUN, lpf=lp)
+                o = sc["orc"].trace_record(f, w, h, reference_rows=bool(ref_rows), sun_dir=sd,
+                                           shadow_flags=B.RTQ_SHADOW_SUN, lpf=lp, threads=16)
+                sel = slice(w, None) if ref_rows else slice(None)
+                bad = np.flatnonzero((g[sel] != o[sel]).any(1))
+                assert bad.size == 0, f"record {w}x{h} rows{ref_rows} lpf{use_lpf}: {bad.size} pixels differ"
+                if ref_rows:
+                    assert (g[:w] == 0xFFFFFFFF).all()  # row 0 is never written
+                hit = o[sel][:, 3].view(np.float32) < 100
+                assert hit.any() and (o[sel][hit, 4] != 0).any()
+                if use_lpf:
+                    assert (o[sel][hit, 5] != 0).any() and (o[sel][hit, 5] != 0xFF).any()
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("blocks", [1, 8])
+def test_dispatch_modes_identical(persistent, blocks):
+    _print(f"case: dispatch persistent={persistent} blocks_per_cu={blocks}, spheres cut + soup bounce rays")
+    d = G.dev()
+    keep = (d.get_option(B.RTQ_OPTION_PERSISTENT), d.get_option(B.RTQ_OPTION_BLOCKS_PER_CU))
+    d.set_option(B.RTQ_OPTION_PERSISTENT, persistent)
+    d.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, blocks)
+    try:
+        s = G.scene(2, 1 / 16)
+        f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 400 / 300)
+        rays = H.frustum_rays(f, 400, 300, seed=7)
+        G.assert_closest_equal(s["gpu"].trace_closest(rays), s["orc"].trace_closest(rays, threads=16), "spheres")
+        rays["tmax"] = np.float32(40.0)
+        assert np.array_equal(s["gpu"].trace_any(rays), s["orc"].trace_any(rays, threads=16)[0])
+        s3 = G.scene(3, 1 / 48)
+        rays3 = s3["builder"].surface_rays(seed=8, count=50_000)
+        G.assert_closest_equal(s3["gpu"].trace_closest(rays3), s3["orc"].trace_closest(rays3, threads=16), "soup")
+        assert np.array_equal(s3["gpu"].trace_any(rays3), s3["orc"].trace_any(rays3, threads=16)[0])
+        # fewer rays than lanes in the grid, and a single ray
+        G.assert_closest_equal(s3["gpu"].trace_closest(rays3[:37]), s3["orc"].trace_closest(rays3[:37], threads=1),
+                               "soup small")
+    finally:
+        d.set_option(B.RTQ_OPTION_PERSISTENT, keep[0])
+        d.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, keep[1])
+
+
 def test_box_scene_any_hit():
     _print("case: closed box any-hit, random t limits")
     s = G.scene(1)

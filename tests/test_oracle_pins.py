@@ -315,7 +315,7 @@ def test_shadow_mask_known_occluder():
     rays["dir"] = [0, 0, -1]
     rays["tmax"] = 100
     o = sc.trace_closest(rays, threads=1)
-    assert np.all(o["tri_id"] >= 2) is False or True
+    assert set(o["tri_id"].tolist()) <= {0, 1}    # the floor quad was added first
     m = sc.shadow_masks(rays, o["t"], o["nid"], (0, 0, -1), B.RTQ_SHADOW_SUN, threads=1)
     # rays start below the occluder at z = 0.5, so their hits are on the floor
     assert list(m & 1) == [1, 1, 0, 0]            # sun (straight down): occluded under x in [0,1]
@@ -326,6 +326,77 @@ def test_shadow_mask_known_occluder():
     o = sc.trace_closest(rays, threads=1)
     assert (o["tri_id"] == B.RTQ_NO_HIT).all()
     assert (sc.shadow_masks(rays, o["t"], o["nid"], (0, 0, -1), 1, threads=1) == 0).all()
+
+
+# ---------------------------------------------------------------- camera-pass record (cameraPass.comp:69-75)
+def _probe_scene():
+    """Floor z=0 over [0,6]x[0,4] and two walls x=2.2, x=3.7 (y in [0,4], z in [0,1])."""
+    b = H.BVHBuilder()
+    quads = (
+        [[0, 0, 0], [6, 0, 0], [0, 4, 0], [6, 4, 0]],
+        [[2.2, 0, 0], [2.2, 4, 0], [2.2, 0, 1], [2.2, 4, 1]],
+        [[3.7, 0, 0], [3.7, 4, 0], [3.7, 0, 1], [3.7, 4, 1]],
+    )
+    for q in quads:
+        off = b.add_vertices(np.array(q, F32))
+        b.add_triangle_list(off, [0, 1, 2, 1, 3, 2])
+    b.build(1)
+    return b
+
+
+@pytest.mark.parametrize("plant", [0, 4, 5])
+def test_record_probe_and_sun_masks_closed_form(plant):
+    # Probe field over [0,4]x[0,4]x[-1,3] with 4 cells per axis: step 1, probes at k + 0.5 (z: k - 0.5).
+    # A floor hit at x lies in cell min(int(max(x - 0.5, 0)), 2) along x; its 8 corner segments
+    # (bit i: x offset i&1, y offset (i>>1)&1, z offset (i>>2)&1) toward the z = -0.5 probes all
+    # cross the floor (bits 0-3 set); toward the z = +0.5 probes they are blocked only by a wall
+    # lying between x and the corner's x. By hand, per x interval:
+    #   [0, 1.5): corners 0.5, 1.5 -> no wall between -> 0x0F
+    #   (1.5, 2.2): corners 1.5, 2.5 -> wall 2.2 before 2.5 -> bits 5, 7 -> 0xAF
+    #   (2.2, 2.5): corners 1.5, 2.5 -> wall 2.2 before 1.5 -> bits 4, 6 -> 0x5F
+    #   (2.5, 3.7): corners 2.5, 3.5 (cell capped at resolution - 2) -> 0x0F
+    #   (3.7, 6]: corners 2.5, 3.5 -> wall 3.7 before both -> 0xFF
+    # Sun travelling along (-1, 0, -1): the shadow ray (1, 0, 1) from the floor meets a 1-high wall at
+    # x_w when 0 < x_w - x < 1, so bit 0 is set exactly for x in (1.2, 2.2) or (2.7, 3.7).
+    b = _probe_scene()
+    blob, r = b.fill_gpu_buffer()
+    sc = oracle.Scene(blob, r, b.positions())
+    f = H.camera_frustum((2.3, 2.0, 5.0), (0, 0, -1), (0, 1, 0), H.REFERENCE_FOV, 1.0)
+    W = Hh = 96
+    s2 = np.float32(np.sqrt(np.float32(0.5)))
+    lpf = ((0, 0, -1), (4, 4, 3), (4, 4, 4))
+    rec = sc.trace_record(f, W, Hh, sun_dir=(-s2, 0, -s2), shadow_flags=B.RTQ_SHADOW_SUN, lpf=lpf, threads=4,
+                          plant=plant)
+    rays = oracle.camera_rays(f, W, Hh)
+    t = rec[:, 3].view(np.float32)
+    hit = t < 100
+    pos = rays["org"] + rays["dir"] * t[:, None]
+    floor = hit & (np.abs(pos[:, 2]) < 1e-3) & (pos[:, 1] > 0.05) & (pos[:, 1] < 3.95)
+    x = pos[:, 0]
+    away = np.ones_like(floor)
+    for edge in (1.2, 1.5, 2.2, 2.5, 2.7, 3.7):
+        away &= np.abs(x - edge) > 0.02
+    sel = floor & away
+    assert sel.sum() > 1500
+    expect = np.select([x < 1.5, x < 2.2, x < 2.5, x < 3.7], [0x0F, 0xAF, 0x5F, 0x0F], 0xFF)
+    assert all((expect[sel] == v).sum() > 50 for v in (0x0F, 0xAF, 0x5F, 0xFF))
+    assert np.array_equal(rec[sel, 5], expect[sel]) == (plant == 0)
+    if plant:
+        return
+    sun = ((x > 1.2) & (x < 2.2)) | ((x > 2.7) & (x < 3.7))
+    assert np.array_equal(rec[sel, 4] & 1, sun[sel].astype(np.uint32))
+    # every floor hit names a floor triangle, whose triple is the first two blob triangles'
+    tris = blob[r.triangles.offset:r.triangles.offset + 12 * 6].view(np.uint32).reshape(6, 3)
+    floor_rows = {tuple(tris[0]), tuple(tris[1])}
+    assert all(tuple(rec[i, :3]) in floor_rows for i in np.flatnonzero(floor))
+    # the record's t is the closest-hit t of the same rays; misses hold zero words and masks
+    o = sc.trace_closest(rays, threads=4)
+    assert np.array_equal(t, o["t"])
+    miss = ~hit
+    assert miss.sum() > 100 and (rec[miss][:, [0, 1, 2, 4, 5, 6, 7]] == 0).all()
+    # without a probe field the probe mask is 0
+    rec0 = sc.trace_record(f, W, Hh, sun_dir=(-s2, 0, -s2), shadow_flags=B.RTQ_SHADOW_SUN, lpf=None, threads=4)
+    assert (rec0[:, 5] == 0).all() and np.array_equal(rec0[:, 4], rec[:, 4])
 
 
 # ---------------------------------------------------------------- tile culling (cullFrustum.glsl:63-153)

@@ -26,6 +26,8 @@ RTQ_TRACE_CLOSEST = 1
 RTQ_TRACE_ANY = 2
 RTQ_SHADOW_SUN = 1
 RTQ_CAMERA_REFERENCE_ROWS = 1
+RTQ_OPTION_PERSISTENT = 1
+RTQ_OPTION_BLOCKS_PER_CU = 2
 RTQ_NO_HIT = 0xFFFFFFFF
 RTQ_SCENE_BOX, RTQ_SCENE_SPHERES, RTQ_SCENE_SOUP = 1, 2, 3
 
@@ -41,6 +43,10 @@ class Ranges(C.Structure):
 class Frustum(C.Structure):
     _fields_ = [("camera_pos", C.c_float * 3), ("corner00", C.c_float * 3),
                 ("corner10", C.c_float * 3), ("corner01", C.c_float * 3)]
+
+
+class Lpf(C.Structure):
+    _fields_ = [("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3), ("resolution", C.c_uint32 * 3)]
 
 
 class BuildParams(C.Structure):
@@ -122,6 +128,8 @@ _DEVICE_SIGS = {
     "rtq_device_free": ([P, P], C.c_int),
     "rtq_copy_to_device": ([P, P, P, C.c_size_t, P], C.c_int),
     "rtq_device_copy": ([P, P, P, C.c_size_t, P], C.c_int),
+    "rtq_device_set_option": ([P, U32, U32], C.c_int),
+    "rtq_device_get_option": ([P, U32, C.POINTER(U32)], C.c_int),
     "rtq_stream_create": ([P, C.POINTER(P)], C.c_int),
     "rtq_stream_destroy": ([P, P], C.c_int),
     "rtq_stream_synchronize": ([P, P], C.c_int),
@@ -136,6 +144,7 @@ _DEVICE_SIGS = {
     "rtq_dispatch_rays_counted": ([P, P, U32, U32, P, P, P, P], C.c_int),
     "rtq_read_hits": ([P, P, P, C.c_size_t, P], C.c_int),
     "rtq_trace_camera": ([P, C.POINTER(Frustum), U32, U32, U32, P, P, P], C.c_int),
+    "rtq_trace_camera_record": ([P, C.POINTER(Frustum), U32, U32, U32, P, U32, C.POINTER(Lpf), P, P], C.c_int),
     "rtq_shadow_masks": ([P, P, P, P, U32, P, U32, P, P], C.c_int),
     "rtq_cull_lights": ([P, C.POINTER(Frustum), U32, U32, P, P, U32, C.POINTER(U32), P], C.c_int),
     "rtq_selftest_division": ([P, U64, U64, C.POINTER(U64)], C.c_int),

@@ -260,6 +260,113 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s, const float
   }
 }
 
+// ---- persistent traversal with lane refill ------------------------------------------------
+// A fixed grid of waves; each lane traces one ray at a time. After every descent-to-leaf step
+// the lanes whose ray has finished take new rays from a device counter (one atomic per wave), so
+// lanes no longer idle while the slowest ray of their wave finishes. Each ray still follows the
+// reference visit order exactly; only the assignment of rays to lanes changes.
+template <int STACK, bool ANY>
+__global__ __launch_bounds__(kTraceBlock) void kTracePersistent(DevScene s, const float4* __restrict__ rays,
+                                                                uint32_t n, void* __restrict__ out,
+                                                                uint32_t* __restrict__ leafNid,
+                                                                uint32_t* __restrict__ counter) {
+  __shared__ uint32_t lds[STACK * kTraceBlock];
+  uint32_t* stk = lds + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  bool active = false, fast = false, exhausted = false;
+  uint32_t rid = 0, node = kRefDone;
+  int sp = 0;
+  float tmax = 0.0f;
+  Ray3 r{};
+  RayRcp rq{};
+  Closest hit{};
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    if (idle && !exhausted) {
+      const uint32_t want = uint32_t(__popcll(idle));
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(counter, want);
+      base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+      if (base + want >= n) exhausted = true;
+      if (!active) {
+        const uint32_t k = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+        if (k < n) {
+          rid = k;
+          r = loadRay(rays, k, tmax);
+          fast = fastLane(s, r, tmax);
+          if (fast) rq = RayRcp{refinedRcp(r.dx), refinedRcp(r.dy), refinedRcp(r.dz)};
+          hit = Closest{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
+          node = s.root;
+          sp = 0;
+          active = true;
+        }
+      }
+    }
+    if (!__any(active)) break;
+    if (!active) continue;
+    bool accepted = false;
+    while (node < kRefDone) {
+      const float4* np = reinterpret_cast<const float4*>(s.nodes + node);
+      const float4 a = np[0], b = np[1], c = np[2];
+      const uint4 d = reinterpret_cast<const uint4*>(np)[3];
+      const float lim = ANY ? tmax : hit.t;
+      float d0, d1;
+      if (fast) {
+        d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+        d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+      } else {
+        d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+        d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+      }
+      const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
+      if (h0 && h1) {
+        const bool n1 = d1 < d0;
+        if (sp < STACK) stk[sp * kTraceBlock] = n1 ? d.x : d.y;
+        ++sp;
+        node = n1 ? d.y : d.x;
+      } else if (h0 || h1) {
+        node = h0 ? d.x : d.y;
+      } else {
+        node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
+      }
+    }
+    if (node != kRefDone) {
+      if (node != kRefEmpty) {
+        uint32_t slot = node & ~kRefLeaf;
+        for (;;) {
+          const float4* tp = reinterpret_cast<const float4*>(s.tris + slot);
+          const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+          float u = 0.0f, v = 0.0f;
+          const float tt = triHit(r, t0, t1, t2, ANY ? tmax : hit.t, u, v);
+          if (tt > 0.0f) {
+            if (ANY) {
+              accepted = true;
+              break;
+            }
+            hit.t = tt - kOffset;
+            hit.tri = __float_as_uint(t2.y);
+            hit.nid = __float_as_uint(t2.z);
+            hit.u = u;
+            hit.v = v;
+          }
+          if (__float_as_uint(t2.w) != 0u) break;
+          ++slot;
+        }
+      }
+      node = (!accepted && sp > 0) ? stk[(--sp) * kTraceBlock] : kRefDone;
+    }
+    if (node == kRefDone) {
+      if (ANY) {
+        static_cast<uint32_t*>(out)[rid] = accepted ? 1u : 0u;
+      } else {
+        static_cast<float4*>(out)[rid] = make_float4(__uint_as_float(hit.tri), hit.t, hit.u, hit.v);
+        if (leafNid) leafNid[rid] = hit.nid;
+      }
+      active = false;
+    }
+  }
+}
+
 // ---- camera rays generated in the kernel --------------------------------------------------
 struct Frustum4 {
   float cam[3], c00[3], c10[3], c01[3];
@@ -312,6 +419,123 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceCamera(DevScene s, Frustum4
   if (leafNid) leafNid[i] = h.nid;
 }
 
+// computeShadowMask (bvhLighting.glsl:143-163) at hit position p of the hit leaf `nid`.
+template <int STACK>
+__device__ __forceinline__ uint32_t shadowMask(const DevScene& s, uint32_t* stk, float px, float py, float pz,
+                                               uint32_t nid, float sunX, float sunY, float sunZ, uint32_t flags) {
+  uint32_t mask = 0;
+  Closest dummy{};
+  Counters c;
+  if (flags & 1u) {
+    const Ray3 sr{px, py, pz, -sunX, -sunY, -sunZ};
+    if (traverseAuto<true, false, STACK>(s, stk, sr, kTMaxF, dummy, c)) mask |= 1u;
+  }
+  const uint32_t node = nid & 0x3FFFFFFFu;
+  const uint32_t off = node < s.numNodes ? s.leafOff[node] : 0xFFFFFFFFu;
+  if (off != 0xFFFFFFFFu && off < s.leafWords) {
+    const uint32_t packed = s.leafData[off];
+    // unpackLightCountAndOffset (bvhLighting.glsl:133-141), NodeTriangleStride 3
+    uint32_t lo = 3u * (packed & 0xFFFu) + ((packed >> 12) & 0xFFu) + ((packed >> 20) & 0xFFu);
+    const uint32_t count = (packed >> 28) & 0xFu;
+    lo += 1u + off;
+    for (uint32_t k = 0; k < count; ++k) {
+      if (lo + k >= s.leafWords) break;
+      const uint32_t li = s.leafData[lo + k];
+      if (li + 1u >= 32u || li >= s.numLights || s.lightKind[li] != 1u) continue;
+      const float4 L = reinterpret_cast<const float4*>(s.lights)[li];
+      const float dx = L.x - px, dy = L.y - py, dz = L.z - pz;
+      const float len = sqrtf((dx * dx + dy * dy) + dz * dz);
+      bool occ = true;
+      if (len < L.w) {
+        const Ray3 lr{px, py, pz, dx / len, dy / len, dz / len};
+        occ = traverseAuto<true, false, STACK>(s, stk, lr, len, dummy, c);
+      }
+      if (occ) mask |= 1u << (li + 1u);
+    }
+  }
+  return mask;
+}
+
+// computeProbMask (bvhLighting.glsl:165-184) with getLightProbCoordUVW / getLightProbPosition
+// (lightprob/lightprobHelpers.glsl:63-77): 8 unnormalised any-hit segments of t limit 1 toward the
+// corners of the probe cell around p.
+template <int STACK>
+__device__ __forceinline__ uint32_t probMask(const DevScene& s, uint32_t* stk, float px, float py, float pz,
+                                             const LpfHeader& lpf) {
+  const float p[3] = {px, py, pz};
+  float step[3], coordf[3];
+  uint32_t coord[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    step[k] = (lpf.mx[k] - lpf.mn[k]) / float(lpf.res[k]);
+    float uvw = ((p[k] - lpf.mn[k]) / step[k]) - 0.5f;
+    uvw = gmax(uvw, 0.0f);
+    uvw = gmin(uvw, float(lpf.res[k]) - 1.0f);
+    coordf[k] = uvw;
+    // int() of NaN is undefined in GLSL; fixed to 0 (the position is finite for every hit t < TMAX)
+    const uint32_t c = uint32_t(coordf[k] == coordf[k] ? int(coordf[k]) : 0);
+    const uint32_t lim = lpf.res[k] - 2u;
+    coord[k] = c < lim ? c : lim;
+  }
+  uint32_t mask = 0;
+  Closest dummy{};
+  Counters cnt;
+  for (uint32_t i = 0; i < 8; ++i) {
+    const uint32_t o[3] = {i & 1u, (i >> 1) & 1u, (i >> 2) & 1u};
+    float d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float probe = lpf.mn[k] + step[k] * (float(coord[k] + o[k]) + 0.5f);
+      d[k] = probe - p[k];
+    }
+    const Ray3 pr{px, py, pz, d[0], d[1], d[2]};
+    if (traverseAuto<true, false, STACK>(s, stk, pr, 1.0f, dummy, cnt)) mask |= 1u << i;
+  }
+  return mask;
+}
+
+// The camera pass's TRACING_STEP output (cameraPass.comp:69-75): per pixel two uvec4,
+// [vertexOffset, index01, index2_matId, floatBits(t)] and [shadowMask, probMask, 0, 0]. A miss keeps
+// t = TMAX and zero triangle words (the shader leaves whatever its shared memory held) and masks 0.
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void kTraceRecord(DevScene s, Frustum4 f, uint32_t W, uint32_t H,
+                                                            uint32_t referenceRows, float sunX, float sunY,
+                                                            float sunZ, uint32_t shadowFlags, LpfHeader lpf,
+                                                            uint32_t useLpf, uint4* __restrict__ records) {
+  __shared__ uint32_t lds[STACK * kTraceBlock];
+  const uint32_t tilesX = (W + 15u) / 16u;
+  const uint32_t b = xcdBlock();
+  const uint32_t bx = b % tilesX, by = b / tilesX;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t x = bx * 16u + wave * 8u + (lane & 7u), y = by * 8u + (lane >> 3);
+  if (x >= W || y >= H) return;
+  uint32_t gy = y;
+  if (referenceRows) {
+    if (y == 0) return;
+    gy = 16u * ((H + 15u) / 16u) - y;
+  }
+  uint32_t* stk = lds + threadIdx.x;
+  const Ray3 r = cameraRay(f, W, H, x, gy);
+  Closest h{kTMaxF, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
+  Counters c;
+  traverseAuto<false, false, STACK>(s, stk, r, kTMaxF, h, c);
+  uint4 r0 = make_uint4(0u, 0u, 0u, __float_as_uint(h.t)), r1 = make_uint4(0u, 0u, 0u, 0u);
+  if (h.tri != 0xFFFFFFFFu) {
+    r0.x = s.triangles[3 * h.tri];
+    r0.y = s.triangles[3 * h.tri + 1];
+    r0.z = s.triangles[3 * h.tri + 2];
+  }
+  // computeAdditionalHitData (baseRaytracingPass.glsl:85-97)
+  if (h.t < kTMaxF && h.nid != 0xFFFFFFFFu) {
+    const float px = r.ox + r.dx * h.t, py = r.oy + r.dy * h.t, pz = r.oz + r.dz * h.t;
+    r1.x = shadowMask<STACK>(s, stk, px, py, pz, h.nid, sunX, sunY, sunZ, shadowFlags);
+    if (useLpf) r1.y = probMask<STACK>(s, stk, px, py, pz, lpf);
+  }
+  const size_t i = size_t(y) * W + x;
+  records[2 * i] = r0;
+  records[2 * i + 1] = r1;
+}
+
 // computeShadowMask (bvhLighting.glsl:143-163) for one closest hit per lane.
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void kShadowMasks(DevScene s, const float4* __restrict__ rays,
@@ -327,39 +551,10 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowMasks(DevScene s, const fl
   const float t = hits[i].y;
   const uint32_t nid = leafNid[i];
   uint32_t mask = 0;
-  uint32_t* stk = lds + threadIdx.x;
-  Closest dummy{};
-  Counters c;
   if (t < kTMaxF && nid != 0xFFFFFFFFu) {
     // hitPos = from + dir * t (baseRaytracingPass.glsl:91)
     const float px = r.ox + r.dx * t, py = r.oy + r.dy * t, pz = r.oz + r.dz * t;
-    if (flags & 1u) {
-      const Ray3 sr{px, py, pz, -sunX, -sunY, -sunZ};
-      if (traverseAuto<true, false, STACK>(s, stk, sr, kTMaxF, dummy, c)) mask |= 1u;
-    }
-    const uint32_t node = nid & 0x3FFFFFFFu;
-    const uint32_t off = node < s.numNodes ? s.leafOff[node] : 0xFFFFFFFFu;
-    if (off != 0xFFFFFFFFu && off < s.leafWords) {
-      const uint32_t packed = s.leafData[off];
-      // unpackLightCountAndOffset (bvhLighting.glsl:133-141), NodeTriangleStride 3
-      uint32_t lo = 3u * (packed & 0xFFFu) + ((packed >> 12) & 0xFFu) + ((packed >> 20) & 0xFFu);
-      const uint32_t count = (packed >> 28) & 0xFu;
-      lo += 1u + off;
-      for (uint32_t k = 0; k < count; ++k) {
-        if (lo + k >= s.leafWords) break;
-        const uint32_t li = s.leafData[lo + k];
-        if (li + 1u >= 32u || li >= s.numLights || s.lightKind[li] != 1u) continue;
-        const float4 L = reinterpret_cast<const float4*>(s.lights)[li];
-        const float dx = L.x - px, dy = L.y - py, dz = L.z - pz;
-        const float len = sqrtf((dx * dx + dy * dy) + dz * dz);
-        bool occ = true;
-        if (len < L.w) {
-          const Ray3 lr{px, py, pz, dx / len, dy / len, dz / len};
-          occ = traverseAuto<true, false, STACK>(s, stk, lr, len, dummy, c);
-        }
-        if (occ) mask |= 1u << (li + 1u);
-      }
-    }
+    mask = shadowMask<STACK>(s, lds + threadIdx.x, px, py, pz, nid, sunX, sunY, sunZ, flags);
   }
   masks[i] = mask;
 }

@@ -47,6 +47,13 @@ struct DevScene {
   uint32_t leafWords;
   uint32_t numLights;
   uint32_t fastBoxes;        // 1 if every inner-node box bound qualifies for the fast slab division
+  const uint32_t* triangles; // the blob's Triangle records (3 words each), indexed by triangle id
+};
+
+// Light-probe field placement as cameraPass.comp builds LightProbFieldHeader (cameraPass.comp:62-66).
+struct LpfHeader {
+  float mn[3], mx[3];
+  uint32_t res[3];
 };
 
 }  // namespace rtq

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -16,13 +17,22 @@
 
 struct rtq_device {
   int ordinal = 0;
+  int numCU = 0;
+  uint32_t persistent = 1;
+  uint32_t blocksPerCU = 8;
 };
+
+// Work counters for persistent dispatches: a small ring so that dispatches in flight on different
+// streams do not share a counter.
+constexpr uint32_t kCounterRing = 64;
 
 struct rtq_scene {
   rtq_device* dev = nullptr;
   rtq::DevScene view{};
   rtq_scene_info info{};
   std::vector<void*> allocations;
+  uint32_t* counters = nullptr;  // kCounterRing words, in `allocations`
+  std::atomic<uint32_t> nextCounter{0};
 };
 
 namespace {
@@ -126,13 +136,40 @@ rtq_status rtq_device_create(int hip_ordinal, rtq_device** out) {
   RTQ_HIP(hipGetDeviceProperties(&prop, hip_ordinal));
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(RTQ_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
-  *out = new rtq_device{hip_ordinal};
+  rtq_device* d = new rtq_device();
+  d->ordinal = hip_ordinal;
+  d->numCU = prop.multiProcessorCount;
+  *out = d;
   return RTQ_OK;
 }
 
 rtq_status rtq_device_destroy(rtq_device* dev) {
   delete dev;
   return RTQ_OK;
+}
+
+rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t value) {
+  if (!dev) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL device");
+  switch (option) {
+    case RTQ_OPTION_PERSISTENT:
+      if (value > 1) return fail(RTQ_ERR_INVALID_ARGUMENT, "persistent is 0 or 1");
+      dev->persistent = value;
+      return RTQ_OK;
+    case RTQ_OPTION_BLOCKS_PER_CU:
+      if (value < 1 || value > 16) return fail(RTQ_ERR_INVALID_ARGUMENT, "blocks per CU in 1..16");
+      dev->blocksPerCU = value;
+      return RTQ_OK;
+  }
+  return fail(RTQ_ERR_INVALID_ARGUMENT, "unknown option");
+}
+
+rtq_status rtq_device_get_option(const rtq_device* dev, uint32_t option, uint32_t* value) {
+  if (!dev || !value) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  switch (option) {
+    case RTQ_OPTION_PERSISTENT: *value = dev->persistent; return RTQ_OK;
+    case RTQ_OPTION_BLOCKS_PER_CU: *value = dev->blocksPerCU; return RTQ_OK;
+  }
+  return fail(RTQ_ERR_INVALID_ARGUMENT, "unknown option");
 }
 
 rtq_status rtq_device_alloc(rtq_device* dev, size_t bytes, void** d_ptr) {
@@ -378,6 +415,13 @@ rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, 
   if ((st = uploadArray(s, leafData, &s->view.leafData)) != RTQ_OK) return bail(st);
   if ((st = uploadArray(s, lights, &s->view.lights)) != RTQ_OK) return bail(st);
   if ((st = uploadArray(s, kinds, &s->view.lightKind)) != RTQ_OK) return bail(st);
+  {
+    const uint32_t* dc = nullptr;
+    if ((st = uploadArray(s, std::vector<uint32_t>(kCounterRing, 0u), &dc)) != RTQ_OK) return bail(st);
+    s->counters = const_cast<uint32_t*>(dc);
+  }
+  if ((st = uploadArray(s, std::vector<uint32_t>(tris, tris + 3 * size_t(numTris)), &s->view.triangles)) != RTQ_OK)
+    return bail(st);
   s->view.root = rootIsLeaf ? leafSlot[0] : 0u;
   s->view.numNodes = numNodes;
   s->view.leafWords = leafWords;
@@ -414,6 +458,22 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(rays);
+  if (!counters && s->dev->persistent) {
+    uint32_t* counter = s->counters + (s->nextCounter.fetch_add(1) % kCounterRing);
+    hipLaunchKernelGGL(rtq::kResetCounter, dim3(1), dim3(64), 0, st, counter);
+    const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
+    const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, n + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
+    withStack(s->info.max_depth, [&](auto S) {
+      constexpr int K = decltype(S)::value;
+      if (flags == RTQ_TRACE_CLOSEST)
+        hipLaunchKernelGGL((rtq::kTracePersistent<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid, counter);
+      else
+        hipLaunchKernelGGL((rtq::kTracePersistent<K, true>), pgrid, block, 0, st, s->view, r, n, out, nullptr,
+                           counter);
+    });
+    RTQ_HIP(hipGetLastError());
+    return RTQ_OK;
+  }
   withStack(s->info.max_depth, [&](auto S) {
     constexpr int K = decltype(S)::value;
     if (flags == RTQ_TRACE_CLOSEST) {
@@ -461,6 +521,39 @@ rtq_status rtq_trace_camera(rtq_scene* s, const rtq_frustum* f, uint32_t width, 
     constexpr int K = decltype(S)::value;
     hipLaunchKernelGGL((rtq::kTraceCamera<K>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, fr, width,
                        height, ref, h, d_leaf_nid);
+  });
+  RTQ_HIP(hipGetLastError());
+  return RTQ_OK;
+}
+
+rtq_status rtq_trace_camera_record(rtq_scene* s, const rtq_frustum* f, uint32_t width, uint32_t height,
+                                   uint32_t flags, const float sun_dir[3], uint32_t shadowFlags, const rtq_lpf* lpf,
+                                   uint32_t* d_records, void* stream) {
+  if (!s || !f) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene or frustum");
+  if (width == 0 || height == 0) return RTQ_OK;
+  if (!d_records) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL record buffer");
+  if (flags & ~RTQ_CAMERA_REFERENCE_ROWS) return fail(RTQ_ERR_INVALID_ARGUMENT, "bad flags");
+  if ((shadowFlags & RTQ_SHADOW_SUN) && !sun_dir) return fail(RTQ_ERR_INVALID_ARGUMENT, "sun flag without direction");
+  if (width > (1u << 16) || height > (1u << 16)) return fail(RTQ_ERR_LIMIT, "frame larger than 65536 pixels a side");
+  rtq::LpfHeader h{};
+  if (lpf) {
+    std::memcpy(&h, lpf, sizeof(h));
+    for (int k = 0; k < 3; ++k)
+      if (h.res[k] < 2) return fail(RTQ_ERR_INVALID_ARGUMENT, "probe field resolution below 2");
+  }
+  float sd[3] = {0, 0, 0};
+  if (sun_dir) std::memcpy(sd, sun_dir, sizeof(sd));
+  RTQ_HIP(hipSetDevice(s->dev->ordinal));
+  rtq::Frustum4 fr;
+  std::memcpy(&fr, f, sizeof(fr));
+  const uint32_t blocks = ((width + 15u) / 16u) * ((height + 7u) / 8u);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint4* rec = reinterpret_cast<uint4*>(d_records);
+  const uint32_t ref = flags & RTQ_CAMERA_REFERENCE_ROWS;
+  withStack(s->info.max_depth, [&](auto S) {
+    constexpr int K = decltype(S)::value;
+    hipLaunchKernelGGL((rtq::kTraceRecord<K>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, fr, width, height,
+                       ref, sd[0], sd[1], sd[2], shadowFlags, h, lpf ? 1u : 0u, rec);
   });
   RTQ_HIP(hipGetLastError());
   return RTQ_OK;

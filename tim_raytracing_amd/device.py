@@ -105,6 +105,14 @@ class Device:
     def stream(self) -> Stream:
         return Stream(self)
 
+    def set_option(self, option: int, value: int):
+        B.check_device("rtq_device_set_option", self.lib.rtq_device_set_option(self.handle, option, value))
+
+    def get_option(self, option: int) -> int:
+        v = C.c_uint32()
+        B.check_device("rtq_device_get_option", self.lib.rtq_device_get_option(self.handle, option, C.byref(v)))
+        return v.value
+
     def event(self) -> Event:
         return Event(self)
 
@@ -312,6 +320,24 @@ class Scene:
         hits = d_h.read(B.HIT_DTYPE, n)
         return {"tri_id": hits["tri_id"].copy(), "t": hits["t"].copy(), "u": hits["u"].copy(),
                 "v": hits["v"].copy(), "nid": d_n.read(np.uint32, n), "hits": hits}
+
+    def trace_record(self, frustum: B.Frustum, width: int, height: int, flags: int = 0, sun_dir=None,
+                     shadow_flags: int = 0, lpf=None) -> np.ndarray:
+        """The camera pass's TRACING_STEP record, (H*W, 8) uint32. lpf = (min3, max3, res3) or None.
+        Pixels the pass does not write (row 0 with reference rows) read back as 0xFFFFFFFF."""
+        n = width * height
+        d_r = self.dev.alloc(32 * max(n, 1))
+        d_r.write(np.full(8 * n, 0xFFFFFFFF, np.uint32))
+        sd = np.asarray(sun_dir if sun_dir is not None else (0, 0, 0), np.float32)
+        lp = None
+        if lpf is not None:
+            lp = B.Lpf((C.c_float * 3)(*lpf[0]), (C.c_float * 3)(*lpf[1]), (C.c_uint32 * 3)(*lpf[2]))
+        B.check_device("rtq_trace_camera_record",
+                       self.dev.lib.rtq_trace_camera_record(self.handle, C.byref(frustum), width, height, flags,
+                                                            sd.ctypes.data if sun_dir is not None else None,
+                                                            shadow_flags, C.byref(lp) if lp is not None else None,
+                                                            d_r.ptr, None))
+        return d_r.read(np.uint32, 8 * n).reshape(n, 8)
 
     def trace_any(self, rays: np.ndarray, counted: bool = False):
         n = rays.size

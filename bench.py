@@ -48,8 +48,55 @@ def parse():
     ap.add_argument("--distinct", type=int, default=0,
                     help="batch/serve: number of distinct BVHs built; scene slots reuse them round-robin "
                          "(0 = one build per slot)")
-    ap.add_argument("--requests", type=int, default=240, help="serve: requests in the stream")
+    ap.add_argument("--requests", type=int, default=1000, help="serve: requests in the stream")
+    ap.add_argument("--max-batch-rays", type=int, default=4_000_000,
+                    help="serve: rays per served launch (0 = no cap)")
+    ap.add_argument("--max-inflight", type=int, default=0, help="serve: launches on the device at once (0 = "
+                                                                "library default)")
+    ap.add_argument("--dispatch", choices=("auto", "persistent", "grid"), default="auto",
+                    help="lane-refilling persistent launch or one lane per ray; auto: grid for the coherent "
+                         "primary rays, persistent for incoherent bounces")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent dispatch: blocks per CU (0 = library "
+                                                                 "default)")
     return ap.parse_args()
+
+
+def stack_class(depth: int) -> int:
+    return 24 if depth <= 24 else (32 if depth <= 32 else 64)
+
+
+def configure_dispatch(dev, args, auto: str):
+    """Apply --dispatch / --blocks-per-cu to the device; returns a label for the output."""
+    from tim_raytracing_amd import bindings as B
+
+    if args.dispatch == "auto":
+        args.dispatch = auto
+    dev.set_option(B.RTQ_OPTION_PERSISTENT, 1 if args.dispatch == "persistent" else 0)
+    if args.blocks_per_cu:
+        dev.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, args.blocks_per_cu)
+    return {"mode": args.dispatch, "blocks_per_cu": dev.get_option(B.RTQ_OPTION_BLOCKS_PER_CU)}
+
+
+def kernel_name(args, depth: int, any_hit: bool = False) -> str:
+    base = "kTracePersistent" if args.dispatch == "persistent" else ("kTraceAny" if any_hit else "kTraceClosest")
+    flag = ("true" if any_hit else "false") if args.dispatch == "persistent" else "false"
+    return f"{base}<{stack_class(depth)},{flag}>"
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch from the committed PMC summary of this workload and kernel, if present:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the same counters read for a copy of
+    known size."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{workload}.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        rec = json.load(fh)
+    k = rec.get("kernels", {}).get(kernel)
+    if not k:
+        return None, None
+    b = 1024.0 * (k["fetch_kib"] * rec["fetch_scale"] + k["write_kib"] * rec["write_scale"])
+    return int(b), os.path.relpath(path, ROOT)
 
 
 def host_threads(req: int) -> int:
@@ -89,7 +136,7 @@ def cpu_baseline(blob, ranges, pos, rays, threads: int, target_s: float):
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample.size} of the {rays.size} primary rays (every {stride}th) x {passes} passes, closest "
+        "sample": f"{sample.size} of the {rays.size} primary rays (stride {stride}) x {passes} passes, closest "
                   f"hit, bitstack restatement of traverseBvh, {threads} threads, {dt:.1f} s",
     }
 
@@ -124,8 +171,11 @@ def main():
     setup_s = time.perf_counter() - t0
 
     dev = D.Device(0)
+    dispatch_cfg = configure_dispatch(dev, args, "grid")
     scene = D.Scene(dev, blob, ranges, pos)
     info = scene.info()
+    kname = kernel_name(args, int(info.max_depth))
+    traffic, traffic_src = measured_traffic("spheres-1M-primary-1920x1080", kname)
     stream = dev.stream()
     sptr = stream.ptr
     d_rays = dev.upload(rays)
@@ -196,8 +246,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "kTraceClosest<32,false>",
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "kernel": kname,
+            "dispatch": dispatch_cfg,
             "algorithmic_bytes_per_launch": int(alg_bytes),
             "mean_bytes_per_ray": round(alg_bytes / n, 1),
             "mean_inner_nodes": round(sum_inner / n, 2),
@@ -242,9 +294,30 @@ def bench_copy(args):
         copy()
     ev1.record(stream.ptr)
     ms = ev0.elapsed_ms(ev1) / args.steps
+    # link rates with page-locked host memory (what served requests move per ray)
+    hbytes = 256 << 20
+    host = dev.host_alloc(hbytes)
+    host.view(np.uint8)[:] = 1
+
+    def link(direction):
+        fn = (lambda: B.check_device("rtq_copy_to_device", dev.lib.rtq_copy_to_device(
+            dev.handle, src.ptr, host.ptr, hbytes, stream.ptr))) if direction == "h2d" else \
+            (lambda: B.check_device("rtq_read_hits", dev.lib.rtq_read_hits(dev.handle, src.ptr, host.ptr, hbytes,
+                                                                           stream.ptr)))
+        fn()
+        stream.synchronize()
+        ev0.record(stream.ptr)
+        for _ in range(5):
+            fn()
+        ev1.record(stream.ptr)
+        return round(5 * hbytes / (ev0.elapsed_ms(ev1) * 1e-3) / 1e9, 1)
+
+    h2d, d2h = link("h2d"), link("d2h")
+    host.free()
     print(json.dumps({"metric": "device copy GB/s (read + write)", "value": round(2 * nbytes / (ms * 1e-3) / 1e9, 1),
                       "unit": "GB/s", "bytes_copied": nbytes, "ms_per_copy": round(ms, 4), "kernel": "kCopy16",
-                      "steps": args.steps, "warmup": args.warmup}), flush=True)
+                      "steps": args.steps, "warmup": args.warmup,
+                      "pinned_link_gb_s": {"h2d": h2d, "d2h": d2h, "bytes": hbytes}}), flush=True)
 
 
 def _loguniform(rng, lo, hi, n):
@@ -342,7 +415,9 @@ def bench_batch(args):
 def bench_serve(args):
     """Served form: 64 resident scenes of LogU[10K, 3M] triangles, a Poisson stream of requests of
     LogU[1K, 2M] rays (half closest, half any hit) at 1.5x the plain capacity; continuous batching vs
-    one launch per request under the same HBM budget."""
+    one launch per request under the same HBM budget. Request rays and results live in page-locked
+    host memory, so the link moves them by DMA. Both forms are also run saturated (every request
+    submitted at once) to compare their capacities."""
     import threading
 
     from tim_raytracing_amd import bindings as B
@@ -358,17 +433,27 @@ def bench_serve(args):
     build_s = time.perf_counter() - t0
     dev = D.Device(0)
     scenes = [D.Scene.from_builder(dev, builders[i % distinct]) for i in range(slots)]
+    max_depth = max(int(s.info().max_depth) for s in scenes)
     pool_rays = 1 << 21
-    pools = [builders[i].surface_rays(seed=50 + i, count=pool_rays) for i in range(distinct)]
+    pinned_pools = []
+    for i in range(distinct):
+        hb = dev.host_alloc(32 * pool_rays)
+        hb.view(B.RAY_DTYPE, pool_rays)[:] = builders[i].surface_rays(seed=50 + i, count=pool_rays)
+        pinned_pools.append(hb)
+    pools = [hb.view(B.RAY_DTYPE, pool_rays) for hb in pinned_pools]
     nreq = args.requests
     req_sizes = _loguniform(rng, 1000, 2e6, nreq).astype(np.int64)
     req_scene = rng.integers(0, slots, nreq)
     req_any = rng.random(nreq) < 0.5
     req_off = [int(rng.integers(0, pool_rays - n + 1)) for n in req_sizes]
+    out_off = np.concatenate([[0], np.cumsum(np.where(req_any, 4, 16) * req_sizes)]).astype(np.int64)
+    out_arena = dev.host_alloc(int(out_off[-1]))
     budget = 2 << 30
+    params = B.BatchParams(128, args.blocks_per_cu or 8, stack_class(max_depth), 0)
 
     def run(plain, arrivals):
-        srv = D.Server(dev, scenes, budget_bytes=budget, plain=plain)
+        srv = D.Server(dev, scenes, budget_bytes=budget, plain=plain, max_batch_rays=args.max_batch_rays,
+                       params=params, max_inflight=args.max_inflight)
         handles = [None] * nreq
         start = time.perf_counter()
 
@@ -379,9 +464,14 @@ def bench_serve(args):
                     if wait > 0:
                         time.sleep(wait)
                 si = int(req_scene[k])
-                rays = pools[si % distinct][req_off[k]:req_off[k] + int(req_sizes[k])]
-                flags = B.RTQ_TRACE_ANY if req_any[k] else B.RTQ_TRACE_CLOSEST
-                handles[k] = srv.submit(si, flags, rays)
+                n = int(req_sizes[k])
+                rays = pools[si % distinct][req_off[k]:req_off[k] + n]
+                if req_any[k]:
+                    out = out_arena.view(np.uint32, n, int(out_off[k]))
+                    handles[k] = srv.submit(si, B.RTQ_TRACE_ANY, rays, out=out)
+                else:
+                    out = out_arena.view(B.HIT_DTYPE, n, int(out_off[k]))
+                    handles[k] = srv.submit(si, B.RTQ_TRACE_CLOSEST, rays, out=out, with_nid=False)
 
         th = threading.Thread(target=client)
         th.start()
@@ -400,9 +490,10 @@ def bench_serve(args):
                 "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                 "makespan_s": round(end - start, 3), "launches": batches}
 
-    burst = run(True, None)  # plain capacity: everything submitted at once
-    capacity = burst["mrays_s"] * 1e6
-    rate = 1.5 * capacity / float(req_sizes.mean())
+    run(False, None)  # warm-up: first launches of every kernel and scene
+    plain_cap = run(True, None)    # capacities: everything submitted at once
+    served_cap = run(False, None)
+    rate = 1.5 * plain_cap["mrays_s"] * 1e6 / float(req_sizes.mean())
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     served = run(False, arrivals)
     plain = run(True, arrivals)
@@ -412,11 +503,17 @@ def bench_serve(args):
         "data": "synthetic",
         "config": {"workload": "serve-64-scenes-poisson", "scene_slots": slots, "distinct_bvhs": distinct,
                    "triangles_per_slot": "LogU[1e4, %g]" % (3e6 * args.scale), "requests": nreq,
-                   "rays_per_request": "LogU[1000, 2e6]", "mix": "50% closest, 50% any",
-                   "arrival_rate_per_s": round(rate, 2), "offered_load": "1.5x plain capacity",
-                   "hbm_budget_bytes": budget},
-        "served": served, "plain": plain, "plain_burst_capacity": burst,
+                   "rays": int(req_sizes.sum()), "rays_per_request": "LogU[1000, 2e6]",
+                   "mix": "50% closest, 50% any", "arrival_rate_per_s": round(rate, 2),
+                   "offered_load": "1.5x plain capacity", "hbm_budget_bytes": budget,
+                   "host_buffers": "page-locked", "max_batch_rays": args.max_batch_rays,
+                   "max_inflight": args.max_inflight or 3,
+                   "params": {"chunk_rays": params.chunk_rays, "blocks_per_cu": params.blocks_per_cu,
+                              "stack_depth": params.stack_depth}},
+        "served": served, "plain": plain,
         "speedup_vs_plain": round(served["mrays_s"] / plain["mrays_s"], 3) if plain["mrays_s"] else None,
+        "capacity": {"served": served_cap, "plain": plain_cap,
+                     "speedup_vs_plain": round(served_cap["mrays_s"] / plain_cap["mrays_s"], 3)},
         "host": {"threads": threads, "build_s": round(build_s, 1)},
     }), flush=True)
 
@@ -433,6 +530,7 @@ def bench_incoherent(args):
     b, f, W, Hh = H.workload(3, scale=args.scale, threads=threads)
     build_s = time.perf_counter() - t0
     dev = D.Device(0)
+    dispatch_cfg = configure_dispatch(dev, args, "persistent")
     scene = D.Scene.from_builder(dev, b)
     n = 4 << 20
     rays1 = b.surface_rays(seed=3, count=n)
@@ -475,6 +573,7 @@ def bench_incoherent(args):
         "bounce1_hit_rate": round(float((hits1["tri_id"] != B.RTQ_NO_HIT).mean()), 4),
         "bounce2_hit_rate": round(float((hits2["tri_id"] != B.RTQ_NO_HIT).mean()), 4),
         "cull_ms_incl_host_sync": round(cull_ms, 3), "tile_light_pairs": int(lists.size),
+        "dispatch": dispatch_cfg, "kernel": kernel_name(args, int(scene.info().max_depth)),
         "host": {"threads": threads, "build_s": round(build_s, 1)},
     }), flush=True)
 

@@ -106,9 +106,10 @@ rtq_status rtq_device_create(int hip_ordinal, rtq_device** out);
 rtq_status rtq_device_destroy(rtq_device* dev);
 
 /* Dispatch tuning. RTQ_OPTION_PERSISTENT: 1 = rtq_dispatch_rays uses a fixed grid of waves that
- * refill finished lanes with new rays (results identical; default 1), 0 = one lane per ray over a
- * grid of ceil(n/128) blocks. RTQ_OPTION_BLOCKS_PER_CU: persistent blocks of 128 lanes per compute
- * unit (1..16, default 8). */
+ * refill finished lanes with new rays (results identical; faster for incoherent rays such as diffuse
+ * bounces), 0 = one lane per ray over a grid of ceil(n/128) blocks (default; faster for coherent
+ * primary rays, whose neighbouring lanes share nodes). RTQ_OPTION_BLOCKS_PER_CU: persistent blocks of
+ * 128 lanes per compute unit (1..16, default 8). */
 #define RTQ_OPTION_PERSISTENT 1u
 #define RTQ_OPTION_BLOCKS_PER_CU 2u
 rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t value);
@@ -118,6 +119,10 @@ rtq_status rtq_device_get_option(const rtq_device* dev, uint32_t option, uint32_
 rtq_status rtq_device_alloc(rtq_device* dev, size_t bytes, void** d_ptr);
 rtq_status rtq_device_free(rtq_device* dev, void* d_ptr);
 rtq_status rtq_copy_to_device(rtq_device* dev, void* d_dst, const void* h_src, size_t bytes, void* stream);
+/* Page-locked host memory (hipHostMalloc): request rays and results in it move by DMA at full link
+ * rate instead of through the runtime's staging copies of pageable memory. */
+rtq_status rtq_host_alloc(rtq_device* dev, size_t bytes, void** h_ptr);
+rtq_status rtq_host_free(rtq_device* dev, void* h_ptr);
 
 /* Streams and events of the library's HIP runtime, for callers that need ordering and timing
  * without a second runtime in the process. Streams have the default priority. */
@@ -249,8 +254,8 @@ rtq_status rtq_batch_autotune(rtq_batch* batch, uint32_t reps, rtq_batch_params*
 /* ---- served form: resident scenes, budgeted admission, continuous batching ----
  * No counterpart in the reference. Scenes stay resident; requests (host rays in, host results
  * out) are admitted while their device buffers fit the HBM budget, batched with whatever else is
- * admitted, and traced by the persistent batch kernel; up to two batches run at once so new work
- * starts while the previous batch drains. Traversal arithmetic is the same as rtq_dispatch_rays, so
+ * admitted, and traced by the persistent batch kernel; up to max_inflight batches are on the device
+ * at once, so the rays of one batch cross the link while another is traced. Traversal arithmetic is the same as rtq_dispatch_rays, so
  * each result is identical whether the request runs alone or in a mix. rtq_server_submit and
  * rtq_request_* are thread-safe. */
 typedef struct rtq_server rtq_server;
@@ -261,7 +266,7 @@ typedef struct rtq_server_config {
   uint32_t max_queue;        /* waiting requests beyond this make submit return RTQ_ERR_QUEUE_FULL */
   uint32_t max_batch_rays;   /* 0: no cap besides the budget */
   uint32_t plain;            /* 1: one launch per request, no batching (comparison mode) */
-  uint32_t reserved;
+  uint32_t max_inflight;     /* batches (or plain launches) on the device at once, 1..8; 0: 3 */
   rtq_batch_params params;   /* persistent-kernel settings, e.g. from rtq_batch_autotune */
 } rtq_server_config;
 

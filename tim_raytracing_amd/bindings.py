@@ -72,7 +72,7 @@ class BatchParams(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_queue", C.c_uint32), ("max_batch_rays", C.c_uint32),
-                ("plain", C.c_uint32), ("reserved", C.c_uint32), ("params", BatchParams)]
+                ("plain", C.c_uint32), ("max_inflight", C.c_uint32), ("params", BatchParams)]
 
 
 class RequestStats(C.Structure):
@@ -126,6 +126,8 @@ _DEVICE_SIGS = {
     "rtq_device_destroy": ([P], C.c_int),
     "rtq_device_alloc": ([P, C.c_size_t, C.POINTER(P)], C.c_int),
     "rtq_device_free": ([P, P], C.c_int),
+    "rtq_host_alloc": ([P, C.c_size_t, C.POINTER(P)], C.c_int),
+    "rtq_host_free": ([P, P], C.c_int),
     "rtq_copy_to_device": ([P, P, P, C.c_size_t, P], C.c_int),
     "rtq_device_copy": ([P, P, P, C.c_size_t, P], C.c_int),
     "rtq_device_set_option": ([P, U32, U32], C.c_int),

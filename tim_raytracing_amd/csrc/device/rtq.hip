@@ -18,7 +18,7 @@
 struct rtq_device {
   int ordinal = 0;
   int numCU = 0;
-  uint32_t persistent = 1;
+  uint32_t persistent = 0;
   uint32_t blocksPerCU = 8;
 };
 
@@ -183,6 +183,20 @@ rtq_status rtq_device_free(rtq_device* dev, void* d_ptr) {
   if (!dev) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL device");
   RTQ_HIP(hipSetDevice(dev->ordinal));
   if (d_ptr) RTQ_HIP(hipFree(d_ptr));
+  return RTQ_OK;
+}
+
+rtq_status rtq_host_alloc(rtq_device* dev, size_t bytes, void** h_ptr) {
+  if (!dev || !h_ptr) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  RTQ_HIP(hipHostMalloc(h_ptr, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+  return RTQ_OK;
+}
+
+rtq_status rtq_host_free(rtq_device* dev, void* h_ptr) {
+  if (!dev) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL device");
+  RTQ_HIP(hipSetDevice(dev->ordinal));
+  if (h_ptr) RTQ_HIP(hipHostFree(h_ptr));
   return RTQ_OK;
 }
 

@@ -92,6 +92,36 @@ class Event:
             pass
 
 
+class HostBuffer:
+    """Page-locked host memory from the library's runtime, viewed as a numpy array."""
+
+    def __init__(self, dev: "Device", nbytes: int):
+        self.dev = dev
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        B.check_device("rtq_host_alloc", dev.lib.rtq_host_alloc(dev.handle, max(1, self.nbytes), C.byref(p)))
+        self.ptr = p.value
+
+    def view(self, dtype, count: int | None = None, offset: int = 0) -> np.ndarray:
+        dt = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dt.itemsize
+        assert offset + count * dt.itemsize <= self.nbytes
+        raw = (C.c_uint8 * (count * dt.itemsize)).from_address(self.ptr + offset)
+        return np.frombuffer(raw, dtype=dt, count=count)
+
+    def free(self):
+        if self.ptr is not None:
+            self.dev.lib.rtq_host_free(self.dev.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Device:
     def __init__(self, ordinal: int = 0):
         self.lib = B.device_lib()
@@ -101,6 +131,9 @@ class Device:
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
+
+    def host_alloc(self, nbytes: int) -> HostBuffer:
+        return HostBuffer(self, nbytes)
 
     def stream(self) -> Stream:
         return Stream(self)
@@ -210,10 +243,11 @@ class Server:
     """Served form (rtq_server_*): resident scenes, HBM-budget admission, continuous batching."""
 
     def __init__(self, dev: Device, scenes, budget_bytes: int, max_queue: int = 0, max_batch_rays: int = 0,
-                 plain: bool = False, params: B.BatchParams | None = None):
+                 plain: bool = False, params: B.BatchParams | None = None, max_inflight: int = 0):
         self.dev = dev
         arr = (C.c_void_p * max(1, len(scenes)))(*[s.handle.value for s in scenes])
-        cfg = B.ServerConfig(budget_bytes, max_queue, max_batch_rays, int(plain), 0, params or B.BatchParams())
+        cfg = B.ServerConfig(budget_bytes, max_queue, max_batch_rays, int(plain), max_inflight,
+                             params or B.BatchParams())
         h = C.c_void_p()
         B.check_device("rtq_server_create", dev.lib.rtq_server_create(dev.handle, arr, len(scenes), C.byref(cfg),
                                                                       C.byref(h)))
@@ -221,12 +255,12 @@ class Server:
         self._scenes = list(scenes)
 
     def submit(self, scene: int, flags: int, rays: np.ndarray, out: np.ndarray | None = None,
-               nid: np.ndarray | None = None) -> Request:
+               nid: np.ndarray | None = None, with_nid: bool = True) -> Request:
         n = rays.size
         rays = np.ascontiguousarray(rays)
         if out is None:
             out = np.zeros(n, B.HIT_DTYPE) if flags == B.RTQ_TRACE_CLOSEST else np.zeros(n, np.uint32)
-        if nid is None and flags == B.RTQ_TRACE_CLOSEST:
+        if nid is None and flags == B.RTQ_TRACE_CLOSEST and with_nid:
             nid = np.zeros(n, np.uint32)
         h = C.c_void_p()
         B.check_device("rtq_server_submit",

@@ -49,6 +49,8 @@ def parse():
                     help="batch/serve: number of distinct BVHs built; scene slots reuse them round-robin "
                          "(0 = one build per slot)")
     ap.add_argument("--requests", type=int, default=1000, help="serve: requests in the stream")
+    ap.add_argument("--cache-dir", default=None, help="batch/serve: on-disk scene cache directory (built scenes are "
+                                                      "stored there and loaded by later runs)")
     ap.add_argument("--max-batch-rays", type=int, default=4_000_000,
                     help="serve: rays per served launch (0 = no cap)")
     ap.add_argument("--max-inflight", type=int, default=0, help="serve: launches on the device at once (0 = "
@@ -324,18 +326,32 @@ def _loguniform(rng, lo, hi, n):
     return np.exp(rng.uniform(np.log(lo), np.log(hi), n))
 
 
-def _scene_pool(sizes, distinct, threads):
-    """Build one sphere-soup BVH per distinct size (UV spheres of 256 triangles, seeds 1000+i)."""
+def _scene_pool(sizes, distinct, threads, cache_dir=None):
+    """One sphere-soup scene per distinct size (UV spheres of 256 triangles, seeds 1000+i): a list of
+    (builder, blob, ranges, positions). With cache_dir, built scenes are stored there under the hash
+    of their inputs and later runs load them instead of building."""
     from tim_raytracing_amd import bindings as B
     from tim_raytracing_amd import host as H
 
-    builders = []
+    pool = []
     for i, tris in enumerate(sizes[:distinct]):
         b = H.BVHBuilder()
         b.generate(B.RTQ_SCENE_SPHERES, 1000 + i, max(1, int(round(tris / 256))))
+        path = None
+        if cache_dir:
+            os.makedirs(cache_dir, exist_ok=True)
+            key = b.input_hash()
+            path = os.path.join(cache_dir, f"{key:016x}.rtqbvh")
+            hit = H.load_cache(path, key)
+            if hit is not None:
+                pool.append((b, hit[0], hit[1], hit[2]))
+                continue
         b.build(threads)
-        builders.append(b)
-    return builders
+        if path:
+            b.save_cache(path)
+        blob, ranges = b.fill_gpu_buffer()
+        pool.append((b, blob, ranges, b.positions()))
+    return pool
 
 
 def _primary_rays(n, seed):
@@ -358,10 +374,11 @@ def bench_batch(args):
     sizes = _loguniform(rng, 1000, 2e6 * args.scale, slots)
     distinct = args.distinct or slots
     t0 = time.perf_counter()
-    builders = _scene_pool(sizes, distinct, threads)
+    pool = _scene_pool(sizes, distinct, threads, args.cache_dir)
+    builders = [p[0] for p in pool]
     build_s = time.perf_counter() - t0
     dev = D.Device(0)
-    scenes = [D.Scene.from_builder(dev, b) for b in builders]
+    scenes = [D.Scene(dev, blob, r, pos) for _, blob, r, pos in pool]
     ray_counts = _loguniform(rng, 4096, 1 << 20, slots).astype(np.int64)
     items, keep = [], []
     total = 0
@@ -402,7 +419,7 @@ def bench_batch(args):
         "higher_is_better": True, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batch-256-scenes", "scene_slots": slots, "distinct_bvhs": distinct,
                    "triangles_per_slot": "LogU[1e3, %g]" % (2e6 * args.scale),
-                   "distinct_triangles": int(sum(b.stats().num_triangles for b in builders)),
+                   "distinct_triangles": int(sum(r.triangles.size // 12 for _, _, r, _ in pool)),
                    "rays": int(total), "rays_per_item": "LogU[4096, 1048576]", "mix": "50% primary, 50% bounce"},
         "plain_mrays_s": round(total / (plain_ms * 1e-3) / 1e6, 2), "plain_ms": round(plain_ms, 4),
         "speedup_vs_plain": round(plain_ms / batched_ms, 3),
@@ -429,10 +446,12 @@ def bench_serve(args):
     sizes = _loguniform(rng, 1e4, 3e6 * args.scale, slots)
     distinct = args.distinct or slots
     t0 = time.perf_counter()
-    builders = _scene_pool(sizes, distinct, threads)
+    pool = _scene_pool(sizes, distinct, threads, args.cache_dir)
+    builders = [p[0] for p in pool]
     build_s = time.perf_counter() - t0
     dev = D.Device(0)
-    scenes = [D.Scene.from_builder(dev, builders[i % distinct]) for i in range(slots)]
+    # every slot is its own resident copy in HBM, also where slots share a distinct BVH
+    scenes = [D.Scene(dev, *pool[i % distinct][1:]) for i in range(slots)]
     max_depth = max(int(s.info().max_depth) for s in scenes)
     pool_rays = 1 << 21
     pinned_pools = []

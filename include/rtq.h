@@ -38,7 +38,8 @@ typedef enum rtq_status {
   RTQ_ERR_QUEUE_FULL = 8, /* serving: admission queue full, retry later */
   RTQ_ERR_CANCELLED = 9,
   RTQ_ERR_TIMEOUT = 10,
-  RTQ_ERR_INTERNAL = 11
+  RTQ_ERR_INTERNAL = 11,
+  RTQ_ERR_CACHE_MISS = 12 /* scene cache: no file, or one built from other inputs */
 } rtq_status;
 
 /* One sub-range of the flattened BVH blob: byte offset and byte size. */
@@ -189,6 +190,17 @@ typedef struct rtq_lpf {
 rtq_status rtq_trace_camera_record(rtq_scene* scene, const rtq_frustum* frustum, uint32_t width, uint32_t height,
                                    uint32_t flags, const float sun_dir[3], uint32_t shadow_flags, const rtq_lpf* lpf,
                                    uint32_t* d_records, void* stream);
+
+/* The light-probe field's TRACING_STEP (genLightProbField.comp:44-69, dispatched by
+ * LightProbFieldPass::traceLightProbField, lightProbFieldPass.cpp:81-100): for each of the
+ * resolution.x*y*z probes (numbered in 4x4x4 tiles, getLightProbCoord) and each of the 64 directions
+ * `dirs` (xyz + pad, the pass's GenLightProbFieldConstants.rays), a closest-hit ray from the probe's
+ * position, recorded as rtq_trace_camera_record does (probe mask 0: the pass runs without
+ * USE_LPF_MASK) at 8 uint32 per sample, sample = probe * 64 + direction. resolution.x and .y must be
+ * at least 4 (the tiling divides by resolution / 4). */
+#define RTQ_PROBE_RAYS 64u
+rtq_status rtq_trace_probe_field(rtq_scene* scene, const rtq_lpf* lpf, const float dirs[4 * 64],
+                                 const float sun_dir[3], uint32_t shadow_flags, uint32_t* d_records, void* stream);
 
 /* read-hits: synchronising copy of `bytes` from device memory written by a dispatch on `stream`. */
 rtq_status rtq_read_hits(rtq_device* dev, const void* d_src, void* h_dst, size_t bytes, void* stream);

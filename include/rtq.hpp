@@ -203,4 +203,36 @@ class RayTracingPass {
   uint32_t* records_ = nullptr;
 };
 
+// LightProbFieldPass::traceLightProbField (lightProbFieldPass.cpp:81-100): the probe field's tracing
+// step, 64 rays per probe, records laid out as the pass's tracing-result buffer.
+class LightProbFieldPass {
+ public:
+  LightProbFieldPass(Device& dev, const rtq_lpf& lpf) : dev_(dev), lpf_(lpf) {
+    samples_ = size_t(lpf.resolution[0]) * lpf.resolution[1] * lpf.resolution[2] * RTQ_PROBE_RAYS;
+    records_ = static_cast<uint32_t*>(dev_.alloc(32 * samples_));
+  }
+  ~LightProbFieldPass() { dev_.free(records_); }
+  LightProbFieldPass(const LightProbFieldPass&) = delete;
+  LightProbFieldPass& operator=(const LightProbFieldPass&) = delete;
+
+  // dirs: 64 x (x, y, z, 0), e.g. from rtq_probe_directions
+  void traceLightProbField(const BVHData& bvh, const float dirs[4 * 64], const float sunDir[3],
+                           uint32_t shadowFlags = RTQ_SHADOW_SUN, void* stream = nullptr) {
+    check(rtq_trace_probe_field(bvh.scene(), &lpf_, dirs, sunDir, shadowFlags, records_, stream),
+          "rtq_trace_probe_field");
+  }
+  std::vector<uint32_t> readRecords(void* stream = nullptr) const {
+    std::vector<uint32_t> out(8 * samples_);
+    check(rtq_read_hits(dev_.handle(), records_, out.data(), out.size() * 4, stream), "rtq_read_hits");
+    return out;
+  }
+  const uint32_t* deviceRecords() const { return records_; }
+
+ private:
+  Device& dev_;
+  rtq_lpf lpf_;
+  size_t samples_ = 0;
+  uint32_t* records_ = nullptr;
+};
+
 }  // namespace rtq

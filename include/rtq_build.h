@@ -97,6 +97,25 @@ rtq_status rtq_surface_rays(const rtq_builder* b, uint64_t seed, uint32_t count,
  * ray with tmax 0 (traced as an immediate miss). */
 rtq_status rtq_bounce_rays(const rtq_builder* b, uint64_t seed, const rtq_ray* rays, const rtq_hit* hits,
                            uint32_t count, float tmax, rtq_ray* out);
+/* ---- on-disk scene cache (no reference counterpart) ----
+ * A built scene's blob, ranges and vertex positions in one file, keyed by a hash of the builder's
+ * inputs (positions, triangle lists, materials, lights, build parameters). */
+typedef struct rtq_cached_scene rtq_cached_scene;
+rtq_status rtq_builder_input_hash(const rtq_builder* b, uint64_t* hash);
+/* The builder must be built. Written through a temporary file renamed into place. */
+rtq_status rtq_cache_save(const rtq_builder* b, const char* path);
+/* expected_hash 0 accepts any key. RTQ_ERR_CACHE_MISS: no readable file, or a valid file for other
+ * inputs; RTQ_ERR_BAD_BLOB: not a cache file of this version, truncated or corrupt (checksum). */
+rtq_status rtq_cache_load(const char* path, uint64_t expected_hash, rtq_cached_scene** out);
+/* Pointers stay valid until rtq_cached_scene_free; they can go straight to rtq_scene_upload. */
+rtq_status rtq_cached_scene_view(const rtq_cached_scene* c, const void** blob, size_t* bytes, rtq_ranges* ranges,
+                                 const float** positions, uint32_t* num_vertices, uint64_t* hash);
+rtq_status rtq_cached_scene_free(rtq_cached_scene* c);
+
+/* `count` (a square, 64 for the probe pass) stratified unit directions on the sphere, 4 floats each
+ * (xyz, 0), as LightProbFieldPass::sampleRays lays them out (lightProbFieldPass.cpp:151-180); the
+ * jitter is seeded here. */
+rtq_status rtq_probe_directions(uint64_t seed, uint32_t count, float* out);
 
 #ifdef __cplusplus
 }

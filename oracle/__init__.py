@@ -12,9 +12,11 @@ Parity pins (what checks each oracle function, see tests/test_oracle_pins.py):
                         invariance under node renumbering; planted tie-break/shrink mistakes fail
   traverse (any)        == (closest raw t < tmax) on every ray; order independence
   shadow masks          scenes with a known occluder / no occluder
-  probe masks           closed forms: a probe field over an open scene (all 0), a wall between the
-                        hit and every probe corner (all 1), one cell split by a wall (the corner
-                        bits on the far side set, the near side clear)
+  probe masks           closed forms: a floor and two walls give, per interval of the hit's x, which
+                        corner segments are blocked; planted mistakes 4 and 5 fail
+  probe field tracing   tiled probe numbering inverted by the reference's getLightProbIndex; axis
+                        rays from probes in a closed box hit the wall at the known distance
+                        (planted mistake 6 fails)
   light culling         closed forms (light on the view axis, light behind the camera, huge radius)
 """
 from __future__ import annotations
@@ -45,6 +47,8 @@ _SIGS = {
     "orc_cull_lights": ([P, U32, P, U32, U32, P, P, C.c_uint64], C.c_uint64),
     "orc_camera_rays": ([P, U32, U32, C.c_int, P], None),
     "orc_trace_record": ([P, C.c_size_t, P, P, U32, P, U32, U32, C.c_int, P, U32, P, P, P, C.c_int], C.c_int),
+    "orc_trace_probes": ([P, C.c_size_t, P, P, U32, P, P, P, P, U32, P, C.c_int], C.c_int),
+    "orc_probe_position": ([P, P, U32, P], None),
 }
 
 _cache = {}
@@ -160,6 +164,20 @@ class Scene:
         assert rc == 0
         return rec
 
+    def trace_probes(self, lpf, dirs, sun_dir=None, shadow_flags: int = 0, threads: int = 8,
+                     plant: int = 0) -> np.ndarray:
+        """The light-probe field's TRACING_STEP records, (probes*64, 8) uint32, sample = probe*64 + ray."""
+        box = np.array(list(lpf[0]) + list(lpf[1]), np.float32)
+        res = np.array(lpf[2], np.uint32)
+        n = int(res.prod())
+        rec = np.zeros((n * 64, 8), np.uint32)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(64, 4)
+        sd = np.asarray(sun_dir if sun_dir is not None else (0, 0, 0), np.float32)
+        rc = lib(plant).orc_trace_probes(_p(self.blob), self.blob.size, _p(self.ranges), _p(self.pos), self.nv, _p(box),
+                                    _p(res), _p(d), _p(sd), shadow_flags, _p(rec), threads)
+        assert rc == 0
+        return rec
+
     def cull_lights(self, frustum, width: int, height: int):
         lights = self.lights()
         return cull_lights(lights, frustum, width, height)
@@ -189,6 +207,14 @@ def camera_rays(frustum, width: int, height: int, reference_rows: bool = False) 
     out = np.zeros(width * height * 8, np.float32)
     lib().orc_camera_rays(_p(fr), width, height, int(reference_rows), _p(out))
     return out.view(np.dtype([("org", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("pad", "<u4")]))
+
+
+def probe_position(lpf, index: int) -> np.ndarray:
+    box = np.array(list(lpf[0]) + list(lpf[1]), np.float32)
+    res = np.array(lpf[2], np.uint32)
+    out = np.zeros(3, np.float32)
+    lib().orc_probe_position(_p(box), _p(res), index, _p(out))
+    return out
 
 
 def tile_planes(frustum, width, height, tx, ty) -> np.ndarray:

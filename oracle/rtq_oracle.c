@@ -35,7 +35,8 @@ typedef uint32_t u32;
 #ifdef ORC_PLANT
 /* Planted mistakes, used only by the tests that check the pins catch them:
  * 1 = tie-break to the far child (d1 <= d0), 2 = no t shrink on acceptance, 3 = u clamp dropped,
- * 4 = probe cell without the half-cell shift, 5 = probe cell not capped at resolution - 2. */
+ * 4 = probe cell without the half-cell shift, 5 = probe cell not capped at resolution - 2,
+ * 6 = probe position without the half-step offset. */
 static const int kPlant = ORC_PLANT;
 #else
 static const int kPlant = 0;
@@ -746,6 +747,95 @@ int orc_trace_record(const uint8_t* blob, size_t bytes, const u32* ranges, const
   run_record(&jobs[0]);
   for (int t = 1; t < threads; ++t) pthread_join(th[t], 0);
   free(rays);
+  view_free(&view);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * The light-probe field's TRACING_STEP (Shaders/genLightProbField.comp:44-69): probe index ->
+ * coordinate by getLightProbCoord with LP_TILE_RES 4 (lightprob/lightprobHelpers.glsl:35-52),
+ * position by getLightProbPosition (:62-68), one closest-hit ray per (probe, direction) from that
+ * position, the record as in orc_trace_record with the probe mask 0 (the pass has no USE_LPF_MASK).
+ * Sample s = probe * 64 + direction. */
+void orc_probe_position(const float* lpf_box, const u32* res, u32 index, float out[3]) {
+  u32 tres[2] = {res[0] / 4u, res[1] / 4u};
+  u32 tile = index / 64u, in_tile = index % 64u;
+  u32 c[3];
+  c[2] = tile / (tres[0] * tres[1]);
+  tile = tile % (tres[0] * tres[1]);
+  c[1] = tile / tres[0];
+  c[0] = tile % tres[0];
+  c[0] = c[0] * 4u + in_tile % 4u;
+  c[1] = c[1] * 4u + (in_tile / 4u) % 4u;
+  c[2] = c[2] * 4u + (in_tile / 16u) % 4u;
+  for (int k = 0; k < 3; ++k) {
+    float dim = lpf_box[3 + k] - lpf_box[k];
+    float step = dim / (float)res[k];
+    out[k] = lpf_box[k] + step * ((float)c[k] + (kPlant == 6 ? 0.0f : 0.5f));
+  }
+}
+
+typedef struct {
+  const orc_view* v;
+  const float* lpf_box;
+  const u32* lpf_res;
+  const float* dirs;
+  const float* sun;
+  u32 shadow_flags;
+  u32* records;
+  u32 begin, end; /* probe range */
+} orc_probe_job;
+
+static void* run_probes(void* arg) {
+  orc_probe_job* j = (orc_probe_job*)arg;
+  const orc_view* v = j->v;
+  for (u32 p = j->begin; p < j->end; ++p) {
+    float o[3];
+    orc_probe_position(j->lpf_box, j->lpf_res, p, o);
+    for (u32 k = 0; k < 64; ++k) {
+      const float* d = j->dirs + 4 * k;
+      orc_hit h;
+      memset(&h, 0, sizeof(h));
+      h.t = TMAX;
+      h.slot = 0xFFFFFFFFu;
+      h.nid = 0xFFFFFFFFu;
+      traverse_bitstack(v, o, d, TMAX, 0, &h);
+      u32* out = j->records + 8 * ((size_t)p * 64 + k);
+      memset(out, 0, 8 * sizeof(u32));
+      if (h.slot != 0xFFFFFFFFu) {
+        out[0] = v->leaf[h.slot];
+        out[1] = v->leaf[h.slot + 1];
+        out[2] = v->leaf[h.slot + 2];
+      }
+      memcpy(&out[3], &h.t, 4);
+      if (h.t < TMAX && h.nid != 0xFFFFFFFFu) {
+        float pos[3] = {o[0] + d[0] * h.t, o[1] + d[1] * h.t, o[2] + d[2] * h.t};
+        out[4] = shadow_mask_at(v, pos, h.nid, j->sun, j->shadow_flags);
+      }
+    }
+  }
+  return 0;
+}
+
+int orc_trace_probes(const uint8_t* blob, size_t bytes, const u32* ranges, const float* pos, u32 nv,
+                     const float* lpf_box, const u32* lpf_res, const float* dirs, const float* sun_dir,
+                     u32 shadow_flags, u32* records, int threads) {
+  if (lpf_res[0] < 4 || lpf_res[1] < 4) return -3;
+  orc_view view;
+  if (view_init(&view, blob, bytes, ranges, pos, nv)) { view_free(&view); return -1; }
+  u32 n = lpf_res[0] * lpf_res[1] * lpf_res[2];
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  pthread_t th[256];
+  orc_probe_job jobs[256];
+  for (int t = 0; t < threads; ++t) {
+    orc_probe_job jj = {&view, lpf_box, lpf_res, dirs, sun_dir, shadow_flags, records,
+                        (u32)((uint64_t)n * t / threads), (u32)((uint64_t)n * (t + 1) / threads)};
+    jobs[t] = jj;
+  }
+  for (int t = 1; t < threads; ++t) pthread_create(&th[t], 0, run_probes, &jobs[t]);
+  run_probes(&jobs[0]);
+  for (int t = 1; t < threads; ++t) pthread_join(th[t], 0);
   view_free(&view);
   return 0;
 }

@@ -16,6 +16,9 @@ int orc_trace_record(const uint8_t* blob, size_t bytes, const uint32_t* ranges, 
                      const float* frustum, uint32_t width, uint32_t height, int reference_rows, const float* sun_dir,
                      uint32_t shadow_flags, const float* lpf_box, const uint32_t* lpf_res, uint32_t* records,
                      int threads);
+int orc_trace_probes(const uint8_t* blob, size_t bytes, const uint32_t* ranges, const float* pos, uint32_t nv,
+                     const float* lpf_box, const uint32_t* lpf_res, const float* dirs, const float* sun_dir,
+                     uint32_t shadow_flags, uint32_t* records, int threads);
 }
 
 int main() {
@@ -88,6 +91,23 @@ int main() {
     if (recBad || masked < n / 8) {
       std::fprintf(stderr, "record mismatches %zu, probe-masked pixels %zu\n", recBad, masked);
       return 1;
+    }
+    // the light-probe field's tracing step through LightProbFieldPass
+    {
+      float dirs[4 * 64];
+      if (rtq_probe_directions(5, 64, dirs) != RTQ_OK) return 3;
+      rtq_lpf field{{0.5f, 0.5f, 0.5f}, {7.5f, 7.5f, 4.5f}, {4, 4, 2}};
+      rtq::LightProbFieldPass lp(dev, field);
+      lp.traceLightProbField(data, dirs, sun);
+      std::vector<uint32_t> prec = lp.readRecords();
+      std::vector<uint32_t> oprec(prec.size(), 0);
+      const float fbox[6] = {0.5f, 0.5f, 0.5f, 7.5f, 7.5f, 4.5f};
+      orc_trace_probes(blob.data(), blob.size(), rr, p.first, p.second, fbox, field.resolution, dirs, sun,
+                       RTQ_SHADOW_SUN, oprec.data(), 4);
+      if (prec != oprec) {
+        std::fprintf(stderr, "probe records differ\n");
+        return 1;
+      }
     }
     // a failing call throws with its status
     try {

@@ -95,6 +95,30 @@ def test_camera_record_matches_oracle():
                     assert (o[sel][hit, 5] != 0).any() and (o[sel][hit, 5] != 0xFF).any()
 
 
+def test_probe_field_tracing_matches_oracle():
+    _print("case: light-probe field TRACING_STEP, closed box 4x4x4 probes + spheres cut 8x8x4 and 12x4x3 probes")
+    from tests.test_oracle_pins import _closed_box
+    b = _closed_box()
+    blob, r = b.fill_gpu_buffer()
+    box = {"gpu": D.Scene(G.dev(), blob, r, b.positions()), "orc": oracle.Scene(blob, r, b.positions())}
+    dirs = H.probe_directions(11)
+    dirs[:6, :3] = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+    sp = G.scene(2, 1 / 16)
+    mn, mx = sp["builder"].scene_box()
+    sun = np.array([0.3, -0.2, -0.93], np.float32)
+    sun /= np.linalg.norm(sun)
+    cases = [(box, ((1, 1, 1), (9, 9, 9), (4, 4, 4)), (0, 0, -1)),
+             (sp, (tuple(mn), tuple(mx), (8, 8, 4)), tuple(sun)),
+             (sp, (tuple(mn), tuple(mx), (12, 4, 3)), tuple(sun))]
+    for sc, lpf, sd in cases:
+        g = sc["gpu"].trace_probe_field(lpf, dirs, sun_dir=sd, shadow_flags=B.RTQ_SHADOW_SUN)
+        o = sc["orc"].trace_probes(lpf, dirs, sun_dir=sd, shadow_flags=B.RTQ_SHADOW_SUN, threads=16)
+        bad = np.flatnonzero((g != o).any(1))
+        assert bad.size == 0, f"probe records differ on {bad.size} samples of {lpf[2]}, first {bad[:5]}"
+        hit = o[:, 3].view(np.float32) < 100
+        assert hit.any() and (o[hit, 4] != 0).any()
+
+
 @pytest.mark.parametrize("persistent", [0, 1])
 @pytest.mark.parametrize("blocks", [1, 8])
 def test_dispatch_modes_identical(persistent, blocks):

@@ -399,6 +399,83 @@ def test_record_probe_and_sun_masks_closed_form(plant):
     assert (rec0[:, 5] == 0).all() and np.array_equal(rec0[:, 4], rec[:, 4])
 
 
+# ---------------------------------------------------------------- light-probe field (genLightProbField.comp)
+def _probe_index(c, res):
+    """getLightProbIndex (lightprob/lightprobHelpers.glsl:16-33, LP_TILE_RES 4): the inverse numbering."""
+    tile = [c[k] // 4 for k in range(3)]
+    tres = [res[k] // 4 for k in range(3)]
+    idx = (tile[0] + tile[1] * tres[0] + tile[2] * tres[0] * tres[1]) * 64
+    w = [c[k] % 4 for k in range(3)]
+    return idx + w[0] + w[1] * 4 + w[2] * 16
+
+
+def test_probe_numbering_is_tiled_and_inverted_by_probe_index():
+    for res in ((4, 4, 4), (8, 8, 4), (12, 8, 8)):
+        lpf = ((0, 0, 0), tuple(float(r) for r in res), res)   # step 1: probe c sits at c + 0.5
+        n = res[0] * res[1] * res[2]
+        seen = set()
+        for i in range(n):
+            p = oracle.probe_position(lpf, i)
+            c = tuple(int(v) for v in (p - 0.5))
+            assert np.array_equal(p, np.array(c, np.float32) + np.float32(0.5))
+            assert all(0 <= c[k] < res[k] for k in range(3))
+            assert _probe_index(c, res) == i
+            seen.add(c)
+        assert len(seen) == n
+    # the first 64 probes fill the first 4x4x4 tile, x fastest
+    lpf = ((0, 0, 0), (8.0, 8.0, 4.0), (8, 8, 4))
+    assert [tuple(int(v) for v in oracle.probe_position(lpf, i) - 0.5) for i in (0, 1, 4, 16, 63, 64)] == \
+        [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (3, 3, 3), (4, 0, 0)]
+
+
+def _closed_box(size=10.0):
+    """Six walls of [0,size]^3, two triangles each, wall k added k-th: -x, +x, -y, +y, -z, +z."""
+    b = H.BVHBuilder()
+    s = size
+    walls = (
+        [[0, 0, 0], [0, s, 0], [0, 0, s], [0, s, s]], [[s, 0, 0], [s, s, 0], [s, 0, s], [s, s, s]],
+        [[0, 0, 0], [s, 0, 0], [0, 0, s], [s, 0, s]], [[0, s, 0], [s, s, 0], [0, s, s], [s, s, s]],
+        [[0, 0, 0], [s, 0, 0], [0, s, 0], [s, s, 0]], [[0, 0, s], [s, 0, s], [0, s, s], [s, s, s]],
+    )
+    for w in walls:
+        off = b.add_vertices(np.array(w, F32))
+        b.add_triangle_list(off, [0, 1, 2, 1, 3, 2])
+    b.build(1)
+    return b
+
+
+@pytest.mark.parametrize("plant", [0, 6])
+def test_probe_field_axis_rays_closed_form(plant):
+    # probes of a 4x4x4 field over [1,9]^3 sit at 2, 4, 6, 8 on each axis; from inside the closed box
+    # [0,10]^3 an axis ray hits the facing wall at the exact distance, stored as t - 1e-5
+    b = _closed_box()
+    blob, r = b.fill_gpu_buffer()
+    sc = oracle.Scene(blob, r, b.positions())
+    lpf = ((1, 1, 1), (9, 9, 9), (4, 4, 4))
+    dirs = H.probe_directions(7)
+    axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], F32)
+    dirs[:6, :3] = axes
+    rec = sc.trace_probes(lpf, dirs, sun_dir=(0, 0, -1), shadow_flags=B.RTQ_SHADOW_SUN, threads=4, plant=plant)
+    tris = blob[r.triangles.offset:r.triangles.offset + 12 * 12].view(np.uint32).reshape(12, 3)
+    wall_of = {tuple(tris[k]): k // 2 for k in range(12)}
+    ok = True
+    for p in range(64):
+        # the same probe's position, restated from the tiling (4x4x4: one tile, x fastest)
+        c = np.array([p % 4, (p // 4) % 4, p // 16], F32)
+        pos = np.float32(1) + np.float32(2) * (c + np.float32(0.5))
+        for k in range(6):
+            row = rec[p * 64 + k]
+            axis, sign = k // 2, (1 if k % 2 == 0 else -1)
+            dist = (np.float32(10) - pos[axis]) if sign > 0 else pos[axis]
+            want_t = np.float32(np.float32(dist) - np.float32(1e-5))
+            wall = 2 * axis + (1 if sign > 0 else 0)
+            ok &= row[3].view(np.float32) == want_t and wall_of.get(tuple(row[:3])) == wall
+    assert ok == (plant == 0)
+    # every ray from inside a closed box hits a wall, and the sun (straight down) is always blocked
+    t = rec[:, 3].view(np.float32)
+    assert (t < 100).all() and ((rec[:, 4] & 1) == 1).all() and (rec[:, 5:] == 0).all()
+
+
 # ---------------------------------------------------------------- tile culling (cullFrustum.glsl:63-153)
 def _light_records(pos_r):
     out = np.zeros((len(pos_r), 14), F32)

@@ -20,6 +20,7 @@ STATUS_NAMES = {
     0: "RTQ_OK", 1: "RTQ_ERR_INVALID_ARGUMENT", 2: "RTQ_ERR_HIP", 3: "RTQ_ERR_OUT_OF_MEMORY",
     4: "RTQ_ERR_LIMIT", 5: "RTQ_ERR_BAD_BLOB", 6: "RTQ_ERR_NO_DEVICE", 7: "RTQ_ERR_BUDGET",
     8: "RTQ_ERR_QUEUE_FULL", 9: "RTQ_ERR_CANCELLED", 10: "RTQ_ERR_TIMEOUT", 11: "RTQ_ERR_INTERNAL",
+    12: "RTQ_ERR_CACHE_MISS",
 }
 
 RTQ_TRACE_CLOSEST = 1
@@ -29,6 +30,8 @@ RTQ_CAMERA_REFERENCE_ROWS = 1
 RTQ_OPTION_PERSISTENT = 1
 RTQ_OPTION_BLOCKS_PER_CU = 2
 RTQ_NO_HIT = 0xFFFFFFFF
+RTQ_PROBE_RAYS = 64
+RTQ_ERR_CACHE_MISS = 12
 RTQ_SCENE_BOX, RTQ_SCENE_SPHERES, RTQ_SCENE_SOUP = 1, 2, 3
 
 
@@ -117,6 +120,13 @@ _BUILD_SIGS = {
     "rtq_frustum_rays": ([C.POINTER(Frustum), U32, U32, U64, F, P], C.c_int),
     "rtq_surface_rays": ([P, U64, U32, F, P], C.c_int),
     "rtq_bounce_rays": ([P, U64, P, P, U32, F, P], C.c_int),
+    "rtq_probe_directions": ([U64, U32, P], C.c_int),
+    "rtq_builder_input_hash": ([P, C.POINTER(U64)], C.c_int),
+    "rtq_cache_save": ([P, C.c_char_p], C.c_int),
+    "rtq_cache_load": ([C.c_char_p, U64, C.POINTER(P)], C.c_int),
+    "rtq_cached_scene_view": ([P, C.POINTER(P), C.POINTER(C.c_size_t), C.POINTER(Ranges), C.POINTER(P),
+                               C.POINTER(U32), C.POINTER(U64)], C.c_int),
+    "rtq_cached_scene_free": ([P], C.c_int),
 }
 
 _DEVICE_SIGS = {
@@ -147,6 +157,7 @@ _DEVICE_SIGS = {
     "rtq_read_hits": ([P, P, P, C.c_size_t, P], C.c_int),
     "rtq_trace_camera": ([P, C.POINTER(Frustum), U32, U32, U32, P, P, P], C.c_int),
     "rtq_trace_camera_record": ([P, C.POINTER(Frustum), U32, U32, U32, P, U32, C.POINTER(Lpf), P, P], C.c_int),
+    "rtq_trace_probe_field": ([P, C.POINTER(Lpf), P, P, U32, P, P], C.c_int),
     "rtq_shadow_masks": ([P, P, P, P, U32, P, U32, P, P], C.c_int),
     "rtq_cull_lights": ([P, C.POINTER(Frustum), U32, U32, P, P, U32, C.POINTER(U32), P], C.c_int),
     "rtq_selftest_division": ([P, U64, U64, C.POINTER(U64)], C.c_int),

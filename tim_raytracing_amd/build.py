@@ -16,7 +16,8 @@ import sys
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
-HOST_SRC = [os.path.join(PKG, "csrc", "host", f) for f in ("bvh_builder.cpp", "scenes.cpp", "capi_build.cpp")]
+HOST_SRC = [os.path.join(PKG, "csrc", "host", f) for f in ("bvh_builder.cpp", "scenes.cpp", "blob_cache.cpp",
+                                                         "capi_build.cpp")]
 DEVICE_SRC = [os.path.join(PKG, "csrc", "device", "rtq.hip")]
 DEVICE_DEPS = [os.path.join(PKG, "csrc", "device", f) for f in ("kernels.h", "layout.h", "batch.inc", "serve.inc", "serve_core.h")]
 

@@ -494,6 +494,32 @@ __device__ __forceinline__ uint32_t probMask(const DevScene& s, uint32_t* stk, f
   return mask;
 }
 
+// One TRACING_STEP record: the closest hit's ClosestHit_getHitData (bvhBindings.glsl:85-92) =
+// [vertexOffset, index01, index2_matId, floatBits(t)] and computeAdditionalHitData
+// (baseRaytracingPass.glsl:85-97) = [shadowMask, probMask, 0, 0] when t < TMAX. A miss keeps t = TMAX
+// and zero triangle words (the shader leaves whatever its shared memory held) and masks 0.
+template <int STACK>
+__device__ __forceinline__ void traceRecord(const DevScene& s, uint32_t* stk, const Ray3& r, float sunX, float sunY,
+                                            float sunZ, uint32_t shadowFlags, const LpfHeader& lpf,
+                                            uint32_t useLpf, uint4& r0, uint4& r1) {
+  Closest h{kTMaxF, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
+  Counters c;
+  traverseAuto<false, false, STACK>(s, stk, r, kTMaxF, h, c);
+  r0 = make_uint4(0u, 0u, 0u, __float_as_uint(h.t));
+  r1 = make_uint4(0u, 0u, 0u, 0u);
+  if (h.tri != 0xFFFFFFFFu) {
+    r0.x = s.triangles[3 * h.tri];
+    r0.y = s.triangles[3 * h.tri + 1];
+    r0.z = s.triangles[3 * h.tri + 2];
+  }
+  if (h.t < kTMaxF && h.nid != 0xFFFFFFFFu) {
+    // hitPos = from + dir * t (baseRaytracingPass.glsl:91)
+    const float px = r.ox + r.dx * h.t, py = r.oy + r.dy * h.t, pz = r.oz + r.dz * h.t;
+    r1.x = shadowMask<STACK>(s, stk, px, py, pz, h.nid, sunX, sunY, sunZ, shadowFlags);
+    if (useLpf) r1.y = probMask<STACK>(s, stk, px, py, pz, lpf);
+  }
+}
+
 // The camera pass's TRACING_STEP output (cameraPass.comp:69-75): per pixel two uvec4,
 // [vertexOffset, index01, index2_matId, floatBits(t)] and [shadowMask, probMask, 0, 0]. A miss keeps
 // t = TMAX and zero triangle words (the shader leaves whatever its shared memory held) and masks 0.
@@ -514,24 +540,61 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceRecord(DevScene s, Frustum4
     if (y == 0) return;
     gy = 16u * ((H + 15u) / 16u) - y;
   }
-  uint32_t* stk = lds + threadIdx.x;
-  const Ray3 r = cameraRay(f, W, H, x, gy);
-  Closest h{kTMaxF, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
-  Counters c;
-  traverseAuto<false, false, STACK>(s, stk, r, kTMaxF, h, c);
-  uint4 r0 = make_uint4(0u, 0u, 0u, __float_as_uint(h.t)), r1 = make_uint4(0u, 0u, 0u, 0u);
-  if (h.tri != 0xFFFFFFFFu) {
-    r0.x = s.triangles[3 * h.tri];
-    r0.y = s.triangles[3 * h.tri + 1];
-    r0.z = s.triangles[3 * h.tri + 2];
-  }
-  // computeAdditionalHitData (baseRaytracingPass.glsl:85-97)
-  if (h.t < kTMaxF && h.nid != 0xFFFFFFFFu) {
-    const float px = r.ox + r.dx * h.t, py = r.oy + r.dy * h.t, pz = r.oz + r.dz * h.t;
-    r1.x = shadowMask<STACK>(s, stk, px, py, pz, h.nid, sunX, sunY, sunZ, shadowFlags);
-    if (useLpf) r1.y = probMask<STACK>(s, stk, px, py, pz, lpf);
-  }
+  uint4 r0, r1;
+  traceRecord<STACK>(s, lds + threadIdx.x, cameraRay(f, W, H, x, gy), sunX, sunY, sunZ, shadowFlags, lpf, useLpf,
+                     r0, r1);
   const size_t i = size_t(y) * W + x;
+  records[2 * i] = r0;
+  records[2 * i + 1] = r1;
+}
+
+// ---- light-probe field tracing step (genLightProbField.comp:44-69 with TRACING_STEP) ----------
+constexpr uint32_t kProbeRays = 64;  // NUM_RAYS_PER_PROB (lightprob/lightprob.glsl)
+struct ProbeDirs {
+  float4 d[kProbeRays];
+};
+
+// getLightProbCoord (lightprob/lightprobHelpers.glsl:35-52, LP_TILE_RES 4): probes are numbered in
+// 4x4x4 tiles, tiles in x, y, z order. getLightProbPosition (:62-68): min + step * (coord + 0.5).
+__device__ __forceinline__ void probePosition(const LpfHeader& lpf, uint32_t index, float& px, float& py,
+                                              float& pz) {
+  const uint32_t tx = lpf.res[0] / 4u, ty = lpf.res[1] / 4u;
+  uint32_t tile = index / 64u;
+  const uint32_t inTile = index % 64u;
+  const uint32_t cz = tile / (tx * ty);
+  tile = tile % (tx * ty);
+  const uint32_t cy = tile / tx, cx = tile % tx;
+  const uint32_t c[3] = {cx * 4u + inTile % 4u, cy * 4u + (inTile / 4u) % 4u, cz * 4u + (inTile / 16u) % 4u};
+  float p[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float step = (lpf.mx[k] - lpf.mn[k]) / float(lpf.res[k]);
+    p[k] = lpf.mn[k] + step * (float(c[k]) + 0.5f);
+  }
+  px = p[0];
+  py = p[1];
+  pz = p[2];
+}
+
+// One wave traces 64 consecutive probes along one direction, as a workgroup of the reference does;
+// the record of ray `rayId` of probe `p` lands at sample p * 64 + rayId. The probe mask is not
+// computed in this pass (the reference dispatches it without USE_LPF_MASK).
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void kTraceProbes(DevScene s, LpfHeader lpf, ProbeDirs dirs,
+                                                            uint32_t numProbes, float sunX, float sunY, float sunZ,
+                                                            uint32_t shadowFlags, uint4* __restrict__ records) {
+  __shared__ uint32_t lds[STACK * kTraceBlock];
+  const uint32_t wave = xcdBlock() * (kTraceBlock / 64u) + (threadIdx.x >> 6);
+  const uint32_t rayId = wave % kProbeRays;
+  const uint32_t probe = (wave / kProbeRays) * 64u + (threadIdx.x & 63u);
+  if (probe >= numProbes) return;
+  float px, py, pz;
+  probePosition(lpf, probe, px, py, pz);
+  const float4 d = dirs.d[rayId];
+  uint4 r0, r1;
+  traceRecord<STACK>(s, lds + threadIdx.x, Ray3{px, py, pz, d.x, d.y, d.z}, sunX, sunY, sunZ, shadowFlags, lpf, 0u,
+                     r0, r1);
+  const size_t i = size_t(probe) * kProbeRays + rayId;
   records[2 * i] = r0;
   records[2 * i + 1] = r1;
 }

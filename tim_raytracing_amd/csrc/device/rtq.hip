@@ -573,6 +573,35 @@ rtq_status rtq_trace_camera_record(rtq_scene* s, const rtq_frustum* f, uint32_t 
   return RTQ_OK;
 }
 
+rtq_status rtq_trace_probe_field(rtq_scene* s, const rtq_lpf* lpf, const float dirs[4 * 64], const float sun_dir[3],
+                                 uint32_t shadowFlags, uint32_t* d_records, void* stream) {
+  if (!s || !lpf || !dirs) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene, probe field or directions");
+  if ((shadowFlags & RTQ_SHADOW_SUN) && !sun_dir) return fail(RTQ_ERR_INVALID_ARGUMENT, "sun flag without direction");
+  rtq::LpfHeader h{};
+  std::memcpy(&h, lpf, sizeof(h));
+  if (h.res[0] < 4 || h.res[1] < 4 || h.res[2] == 0)
+    return fail(RTQ_ERR_INVALID_ARGUMENT, "probe field needs resolution.x, .y >= 4 and .z >= 1");
+  const uint64_t probes = uint64_t(h.res[0]) * h.res[1] * h.res[2];
+  if (probes > (1ull << 24)) return fail(RTQ_ERR_LIMIT, "more than 2^24 probes");
+  if (!d_records) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL record buffer");
+  float sd[3] = {0, 0, 0};
+  if (sun_dir) std::memcpy(sd, sun_dir, sizeof(sd));
+  rtq::ProbeDirs pd;
+  std::memcpy(&pd, dirs, sizeof(pd));
+  RTQ_HIP(hipSetDevice(s->dev->ordinal));
+  // one wave per (64-probe batch, direction), two waves per block
+  const uint64_t waves = ((probes + 63u) / 64u) * rtq::kProbeRays;
+  const uint32_t blocks = uint32_t((waves + 1u) / 2u);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  withStack(s->info.max_depth, [&](auto S) {
+    constexpr int K = decltype(S)::value;
+    hipLaunchKernelGGL((rtq::kTraceProbes<K>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, h, pd,
+                       uint32_t(probes), sd[0], sd[1], sd[2], shadowFlags, reinterpret_cast<uint4*>(d_records));
+  });
+  RTQ_HIP(hipGetLastError());
+  return RTQ_OK;
+}
+
 rtq_status rtq_shadow_masks(rtq_scene* s, const rtq_ray* d_rays, const rtq_hit* d_hits, const uint32_t* d_leaf_nid,
                             uint32_t n, const float sun_dir[3], uint32_t flags, uint32_t* d_masks, void* stream) {
   if (!s) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene");

@@ -68,6 +68,8 @@ class BvhBuilder {
 
   const std::vector<F3>& positions() const { return positions_; }
   const std::vector<TriangleRec>& triangles() const { return triangles_; }
+  const std::vector<MaterialRec>& materials() const { return materials_; }
+  const std::vector<SphereLightDesc>& lights() const { return lights_; }
   u32 numTriangles() const { return u32(triangles_.size()); }
   u32 numLights() const { return u32(lights_.size()); }
   u32 numMaterials() const { return u32(materials_.size()); }

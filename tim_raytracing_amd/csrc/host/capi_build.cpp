@@ -5,12 +5,17 @@
 #include <string>
 
 #include "../../../include/rtq_build.h"
+#include "blob_cache.h"
 #include "bvh_builder.h"
 #include "scenes.h"
 
 struct rtq_builder {
   rtq::BvhBuilder impl;
   bool built = false;
+};
+
+struct rtq_cached_scene {
+  rtq::CachedScene impl;
 };
 
 namespace {
@@ -221,6 +226,65 @@ rtq_status rtq_bounce_rays(const rtq_builder* b, uint64_t seed, const rtq_ray* r
                     reinterpret_cast<const rtq::HitRec*>(hits), count, tmax, reinterpret_cast<rtq::RayRec*>(out));
     return RTQ_OK;
   });
+}
+
+rtq_status rtq_builder_input_hash(const rtq_builder* b, uint64_t* hash) {
+  if (!b || !hash) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  *hash = rtq::inputHash(b->impl);
+  return RTQ_OK;
+}
+
+rtq_status rtq_cache_save(const rtq_builder* b, const char* path) {
+  if (!b || !path) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!b->built) return fail(RTQ_ERR_INVALID_ARGUMENT, "builder not built");
+  return guarded([&] {
+    rtq::saveCache(path, b->impl);
+    return RTQ_OK;
+  });
+}
+
+rtq_status rtq_cache_load(const char* path, uint64_t expected_hash, rtq_cached_scene** out) {
+  if (!path || !out) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  return guarded([&] {
+    rtq::CachedScene c;
+    switch (rtq::loadCache(path, expected_hash, c)) {
+      case 0: break;
+      case 1: return fail(RTQ_ERR_CACHE_MISS, "no readable cache file");
+      case 3: return fail(RTQ_ERR_CACHE_MISS, "cache file built from other inputs");
+      default: return fail(RTQ_ERR_BAD_BLOB, "not a scene cache of this version, truncated or corrupt");
+    }
+    *out = new rtq_cached_scene{std::move(c)};
+    return RTQ_OK;
+  });
+}
+
+rtq_status rtq_cached_scene_view(const rtq_cached_scene* c, const void** blob, size_t* bytes, rtq_ranges* ranges,
+                                 const float** positions, uint32_t* num_vertices, uint64_t* hash) {
+  if (!c) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL cached scene");
+  if (blob) *blob = c->impl.blob.data();
+  if (bytes) *bytes = c->impl.blob.size();
+  if (ranges) {
+    static_assert(sizeof(rtq::BlobRanges) == sizeof(rtq_ranges), "range layout");
+    std::memcpy(ranges, &c->impl.ranges, sizeof(rtq_ranges));
+  }
+  if (positions) *positions = c->impl.positions.empty() ? nullptr : &c->impl.positions[0].x;
+  if (num_vertices) *num_vertices = uint32_t(c->impl.positions.size());
+  if (hash) *hash = c->impl.key;
+  return RTQ_OK;
+}
+
+rtq_status rtq_cached_scene_free(rtq_cached_scene* c) {
+  delete c;
+  return RTQ_OK;
+}
+
+rtq_status rtq_probe_directions(uint64_t seed, uint32_t count, float* out) {
+  if (!out || count == 0) return fail(RTQ_ERR_INVALID_ARGUMENT, "bad arguments");
+  uint32_t side = 1;
+  while (side * side < count) ++side;
+  if (side * side != count) return fail(RTQ_ERR_INVALID_ARGUMENT, "count must be a square");
+  rtq::probeDirections(seed, count, out);
+  return RTQ_OK;
 }
 
 }  // extern "C"

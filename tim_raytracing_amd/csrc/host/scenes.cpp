@@ -353,4 +353,27 @@ void bounceRays(const BvhBuilder& b, u64 seed, const RayRec* rays, const HitRec*
   }
 }
 
+// Stratified uniform directions on the sphere as LightProbFieldPass::sampleRays builds the probe
+// rays (lightProbFieldPass.cpp:151-180): cell (i, j) of a sqrt(count)^2 grid, a jittered point r
+// in it, theta = 2*pi*r.x, phi = acos(1 - 2*r.y), direction (sin phi cos theta, sin phi sin theta,
+// cos phi); index i*sqrt(count) + j. The jitter comes from this library's generator.
+void probeDirections(u64 seed, u32 count, float* out) {
+  u32 side = 1;
+  while (side * side < count) ++side;
+  Rng rng(seed, 0x70726f6265ull);
+  const float kPi = 3.14159265358979f;
+  for (u32 i = 0; i < side; ++i)
+    for (u32 j = 0; j < side; ++j) {
+      const u32 k = i * side + j;
+      const float rx = (float(i) + float(rng.uniform())) / float(side);
+      const float ry = (float(j) + float(rng.uniform())) / float(side);
+      if (k >= count) continue;
+      const float theta = 2.0f * kPi * rx, phi = std::acos(1.0f - 2.0f * ry);
+      out[4 * k] = std::sin(phi) * std::cos(theta);
+      out[4 * k + 1] = std::sin(phi) * std::sin(theta);
+      out[4 * k + 2] = std::cos(phi);
+      out[4 * k + 3] = 0.0f;
+    }
+}
+
 }  // namespace rtq

@@ -27,5 +27,6 @@ void frustumRays(const Frustum& f, u32 width, u32 height, u64 seed, float tmax, 
 void surfaceRays(const BvhBuilder& b, u64 seed, u32 count, float tmax, RayRec* out);
 void bounceRays(const BvhBuilder& b, u64 seed, const RayRec* rays, const HitRec* hits, u32 count, float tmax,
                 RayRec* out);
+void probeDirections(u64 seed, u32 count, float* out);
 
 }  // namespace rtq

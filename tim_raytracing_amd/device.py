@@ -373,6 +373,19 @@ class Scene:
                                                             d_r.ptr, None))
         return d_r.read(np.uint32, 8 * n).reshape(n, 8)
 
+    def trace_probe_field(self, lpf, dirs: np.ndarray, sun_dir=None, shadow_flags: int = 0) -> np.ndarray:
+        """The light-probe field's TRACING_STEP records, (probes*64, 8) uint32; lpf = (min3, max3, res3)."""
+        n = int(np.prod(lpf[2])) * B.RTQ_PROBE_RAYS
+        d_r = self.dev.alloc(32 * max(n, 1))
+        lp = B.Lpf((C.c_float * 3)(*lpf[0]), (C.c_float * 3)(*lpf[1]), (C.c_uint32 * 3)(*lpf[2]))
+        d = np.ascontiguousarray(dirs, np.float32).reshape(B.RTQ_PROBE_RAYS, 4)
+        sd = np.asarray(sun_dir if sun_dir is not None else (0, 0, 0), np.float32)
+        B.check_device("rtq_trace_probe_field",
+                       self.dev.lib.rtq_trace_probe_field(self.handle, C.byref(lp), d.ctypes.data,
+                                                          sd.ctypes.data if sun_dir is not None else None,
+                                                          shadow_flags, d_r.ptr, None))
+        return d_r.read(np.uint32, 8 * n).reshape(n, 8)
+
     def trace_any(self, rays: np.ndarray, counted: bool = False):
         n = rays.size
         d_r = self.dev.upload(rays)

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -110,6 +111,14 @@ class BVHBuilder:
         buf = (C.c_float * (3 * n.value)).from_address(p.value)
         return np.frombuffer(buf, dtype=np.float32).reshape(-1, 3).copy()
 
+    def input_hash(self) -> int:
+        h = C.c_uint64()
+        B.check_build("rtq_builder_input_hash", self._lib.rtq_builder_input_hash(self._h, C.byref(h)))
+        return h.value
+
+    def save_cache(self, path: str):
+        B.check_build("rtq_cache_save", self._lib.rtq_cache_save(self._h, os.fsencode(path)))
+
     def scene_box(self):
         mn = np.zeros(3, np.float32)
         mx = np.zeros(3, np.float32)
@@ -144,6 +153,36 @@ def frustum_rays(f: B.Frustum, width: int, height: int, seed: int, tmax: float =
     lib = B.build_lib()
     out = np.zeros(width * height, dtype=B.RAY_DTYPE)
     B.check_build("rtq_frustum_rays", lib.rtq_frustum_rays(C.byref(f), width, height, seed, tmax, B.ptr(out)))
+    return out
+
+
+def load_cache(path: str, expected_hash: int = 0):
+    """(blob uint8 array, Ranges, positions (n, 3) float32, key) from a scene cache file, or None when
+    the file is missing or was built from other inputs. A corrupt file raises."""
+    lib = B.build_lib()
+    h = C.c_void_p()
+    st = lib.rtq_cache_load(os.fsencode(path), expected_hash, C.byref(h))
+    if st == B.RTQ_ERR_CACHE_MISS:
+        return None
+    B.check_build("rtq_cache_load", st)
+    try:
+        blob_p, nbytes, r = C.c_void_p(), C.c_size_t(), B.Ranges()
+        pos_p, nv, key = C.c_void_p(), C.c_uint32(), C.c_uint64()
+        B.check_build("rtq_cached_scene_view", lib.rtq_cached_scene_view(
+            h, C.byref(blob_p), C.byref(nbytes), C.byref(r), C.byref(pos_p), C.byref(nv), C.byref(key)))
+        blob = np.ctypeslib.as_array((C.c_uint8 * nbytes.value).from_address(blob_p.value)).copy() \
+            if nbytes.value else np.zeros(0, np.uint8)
+        pos = np.ctypeslib.as_array((C.c_float * (3 * nv.value)).from_address(pos_p.value)).reshape(-1, 3).copy() \
+            if nv.value else np.zeros((0, 3), np.float32)
+        return blob, r, pos, key.value
+    finally:
+        lib.rtq_cached_scene_free(h)
+
+
+def probe_directions(seed: int, count: int = 64) -> np.ndarray:
+    """Stratified unit directions (count, 4) as the probe pass samples them."""
+    out = np.zeros((count, 4), np.float32)
+    B.check_build("rtq_probe_directions", B.build_lib().rtq_probe_directions(seed, count, B.ptr(out)))
     return out
 
 

@@ -53,6 +53,7 @@ def parse():
                                                       "stored there and loaded by later runs)")
     ap.add_argument("--max-batch-rays", type=int, default=4_000_000,
                     help="serve: rays per served launch (0 = no cap)")
+    ap.add_argument("--chunk-rays", type=int, default=128, help="serve: rays a wave takes per grab (1 = lane refill)")
     ap.add_argument("--max-inflight", type=int, default=0, help="serve: launches on the device at once (0 = "
                                                                 "library default)")
     ap.add_argument("--dispatch", choices=("auto", "persistent", "grid"), default="auto",
@@ -468,7 +469,7 @@ def bench_serve(args):
     out_off = np.concatenate([[0], np.cumsum(np.where(req_any, 4, 16) * req_sizes)]).astype(np.int64)
     out_arena = dev.host_alloc(int(out_off[-1]))
     budget = 2 << 30
-    params = B.BatchParams(128, args.blocks_per_cu or 8, stack_class(max_depth), 0)
+    params = B.BatchParams(args.chunk_rays, args.blocks_per_cu or 8, stack_class(max_depth), 0)
 
     def run(plain, arrivals):
         srv = D.Server(dev, scenes, budget_bytes=budget, plain=plain, max_batch_rays=args.max_batch_rays,

@@ -242,8 +242,11 @@ typedef struct rtq_batch_item {
   uint32_t* d_leaf_nid;   /* closest hit only, may be NULL */
 } rtq_batch_item;
 
+/* chunk_rays value selecting the lane-refilling kernel: lanes take single rays from any item as soon
+ * as their previous ray is done (best for incoherent rays; chunks keep coherent rays together). */
+#define RTQ_BATCH_REFILL 1u
 typedef struct rtq_batch_params {
-  uint32_t chunk_rays;    /* rays a wave takes per grab: 64, 128, 256 or 512 */
+  uint32_t chunk_rays;    /* rays a wave takes per grab: 64, 128, 256 or 512, or RTQ_BATCH_REFILL */
   uint32_t blocks_per_cu; /* persistent 128-lane blocks per compute unit, 1..16 */
   uint32_t stack_depth;   /* per-lane LDS stack entries: 24, 32 or 64, >= the deepest scene */
   uint32_t use_graph;     /* 1: replay a captured hipGraph (needs a non-NULL stream) */
@@ -258,7 +261,7 @@ rtq_status rtq_batch_get_params(const rtq_batch* batch, rtq_batch_params* params
 rtq_status rtq_batch_launch(rtq_batch* batch, void* stream);
 /* The same items as one plain rtq_dispatch_rays launch each (for comparison). */
 rtq_status rtq_batch_launch_plain(rtq_batch* batch, void* stream);
-/* Times every valid (chunk_rays, blocks_per_cu, stack_depth) on this batch, `reps` launches each,
+/* Times every valid (chunk_rays incl. RTQ_BATCH_REFILL, blocks_per_cu, stack_depth) on this batch, `reps` launches each,
  * keeps the fastest in the batch and returns it with its mean time. Synchronises `stream`. */
 rtq_status rtq_batch_autotune(rtq_batch* batch, uint32_t reps, rtq_batch_params* best, float* best_ms,
                               void* stream);

@@ -51,35 +51,37 @@ def test_serve_mix_equals_alone():
             alone.append((g["hits"].view(np.uint32).copy(), g["nid"]))
         else:
             alone.append((scenes[si].trace_any(rays), None))
-    # mixed: a budget that holds only part of the stream at once, 4 submitting threads
-    srv = D.Server(dev, scenes, budget_bytes=8 << 20)
-    handles = [None] * len(reqs)
+    # mixed: a budget that holds only part of the stream at once, 4 submitting threads; chunked and
+    # lane-refilling batch kernels
+    for params in (None, B.BatchParams(B.RTQ_BATCH_REFILL, 8, 64, 0)):
+        srv = D.Server(dev, scenes, budget_bytes=8 << 20, params=params)
+        handles = [None] * len(reqs)
 
-    def client(t):
-        for k in range(t, len(reqs), 4):
-            si, flags, rays = reqs[k]
-            handles[k] = srv.submit(si, flags, rays)
+        def client(t):
+            for k in range(t, len(reqs), 4):
+                si, flags, rays = reqs[k]
+                handles[k] = srv.submit(si, flags, rays)
 
-    th = [threading.Thread(target=client, args=(t,)) for t in range(4)]
-    for x in th:
-        x.start()
-    for x in th:
-        x.join()
-    batches = set()
-    for k, h in enumerate(handles):
-        assert h.wait() == B.RTQ_OK
-        st = h.stats()
-        batches.add(st.batch_id)
-        assert st.latency_ms >= st.queued_ms >= 0
-        a_out, a_nid = alone[k]
-        if reqs[k][1] == B.RTQ_TRACE_CLOSEST:
-            assert np.array_equal(h.out.view(np.uint32), a_out)
-            assert np.array_equal(h.nid, a_nid)
-        else:
-            assert np.array_equal(h.out, a_out)
-        h.release()
-    assert len(batches) < len(reqs)  # requests were batched together
-    srv.close()
+        th = [threading.Thread(target=client, args=(t,)) for t in range(4)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        batches = set()
+        for k, h in enumerate(handles):
+            assert h.wait() == B.RTQ_OK
+            st = h.stats()
+            batches.add(st.batch_id)
+            assert st.latency_ms >= st.queued_ms >= 0
+            a_out, a_nid = alone[k]
+            if reqs[k][1] == B.RTQ_TRACE_CLOSEST:
+                assert np.array_equal(h.out.view(np.uint32), a_out)
+                assert np.array_equal(h.nid, a_nid)
+            else:
+                assert np.array_equal(h.out, a_out)
+            h.release()
+        assert len(batches) < len(reqs)  # requests were batched together
+        srv.close()
     # plain comparison mode gives the same results
     srv = D.Server(dev, scenes, budget_bytes=8 << 20, plain=True)
     hs = [srv.submit(si, flags, rays) for si, flags, rays in reqs[:10]]

@@ -31,6 +31,7 @@ RTQ_OPTION_PERSISTENT = 1
 RTQ_OPTION_BLOCKS_PER_CU = 2
 RTQ_NO_HIT = 0xFFFFFFFF
 RTQ_PROBE_RAYS = 64
+RTQ_BATCH_REFILL = 1
 RTQ_ERR_CACHE_MISS = 12
 RTQ_SCENE_BOX, RTQ_SCENE_SPHERES, RTQ_SCENE_SOUP = 1, 2, 3
 

@@ -631,7 +631,8 @@ struct BatchItemDev {
   const float4* rays;
   void* out;
   uint32_t* nid;
-  uint64_t pad;
+  uint32_t rayBegin;  // rays of all earlier items
+  uint32_t pad;
 };
 static_assert(sizeof(BatchItemDev) == 48, "BatchItemDev is 48 B");
 
@@ -678,6 +679,127 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* 
           reinterpret_cast<uint32_t*>(it.out)[i] = occ ? 1u : 0u;
         }
       }
+    }
+  }
+}
+
+// Lane-refilling form of the batch: lanes take single rays from the concatenation of all items
+// (one atomic per wave, a binary search over rayBegin per lane), so a lane that finishes early
+// picks up a ray of any item, scene or mode instead of idling until its wave's slowest ray is done.
+// The scene pointers and the closest/any mode are per lane. Each ray still follows the reference
+// visit order exactly; every wave reaches the exit test once the counter passes totalRays.
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __restrict__ scenes,
+                                                            const BatchItemDev* __restrict__ items,
+                                                            uint32_t numItems, uint32_t totalRays,
+                                                            uint32_t* __restrict__ counter) {
+  __shared__ uint32_t lds[STACK * kTraceBlock];
+  uint32_t* stk = lds + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  bool active = false, fast = false, exhausted = false, anyRay = false;
+  uint32_t rid = 0, item = 0, node = kRefDone;
+  int sp = 0;
+  float tmax = 0.0f;
+  const DevNode* nodes = nullptr;
+  const DevTri* tris = nullptr;
+  Ray3 r{};
+  RayRcp rq{};
+  Closest hit{};
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    if (idle && !exhausted) {
+      const uint32_t want = uint32_t(__popcll(idle));
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(counter, want);
+      base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+      if (uint64_t(base) + want >= totalRays) exhausted = true;
+      if (!active) {
+        const uint32_t k = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+        if (k < totalRays) {
+          uint32_t lo = 0, hi = numItems;
+          while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (items[mid].rayBegin <= k) lo = mid;
+            else hi = mid;
+          }
+          item = lo;
+          const BatchItemDev& it = items[lo];
+          const DevScene& s = scenes[it.scene];
+          rid = k - it.rayBegin;
+          nodes = s.nodes;
+          tris = s.tris;
+          r = loadRay(it.rays, rid, tmax);
+          fast = fastLane(s, r, tmax);
+          if (fast) rq = RayRcp{refinedRcp(r.dx), refinedRcp(r.dy), refinedRcp(r.dz)};
+          anyRay = it.flags != 1u;
+          hit = Closest{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};  // any hit keeps t = tmax throughout
+          node = s.root;
+          sp = 0;
+          active = true;
+        }
+      }
+    }
+    if (!__any(active)) break;
+    if (!active) continue;
+    bool accepted = false;
+    while (node < kRefDone) {
+      const float4* np = reinterpret_cast<const float4*>(nodes + node);
+      const float4 a = np[0], b = np[1], c = np[2];
+      const uint4 d = reinterpret_cast<const uint4*>(np)[3];
+      float d0, d1;
+      if (fast) {
+        d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, hit.t);
+        d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, hit.t);
+      } else {
+        d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, hit.t);
+        d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, hit.t);
+      }
+      const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
+      if (h0 && h1) {
+        const bool n1 = d1 < d0;
+        if (sp < STACK) stk[sp * kTraceBlock] = n1 ? d.x : d.y;
+        ++sp;
+        node = n1 ? d.y : d.x;
+      } else if (h0 || h1) {
+        node = h0 ? d.x : d.y;
+      } else {
+        node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
+      }
+    }
+    if (node != kRefDone) {
+      if (node != kRefEmpty) {
+        uint32_t slot = node & ~kRefLeaf;
+        for (;;) {
+          const float4* tp = reinterpret_cast<const float4*>(tris + slot);
+          const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+          float u = 0.0f, v = 0.0f;
+          const float tt = triHit(r, t0, t1, t2, hit.t, u, v);
+          if (tt > 0.0f) {
+            if (anyRay) {
+              accepted = true;
+              break;
+            }
+            hit.t = tt - kOffset;
+            hit.tri = __float_as_uint(t2.y);
+            hit.nid = __float_as_uint(t2.z);
+            hit.u = u;
+            hit.v = v;
+          }
+          if (__float_as_uint(t2.w) != 0u) break;
+          ++slot;
+        }
+      }
+      node = (!accepted && sp > 0) ? stk[(--sp) * kTraceBlock] : kRefDone;
+    }
+    if (node == kRefDone) {
+      const BatchItemDev& it = items[item];
+      if (anyRay) {
+        static_cast<uint32_t*>(it.out)[rid] = accepted ? 1u : 0u;
+      } else {
+        static_cast<float4*>(it.out)[rid] = make_float4(__uint_as_float(hit.tri), hit.t, hit.u, hit.v);
+        if (it.nid) it.nid[rid] = hit.nid;
+      }
+      active = false;
     }
   }
 }

@@ -53,12 +53,13 @@ def parse():
                                                       "stored there and loaded by later runs)")
     ap.add_argument("--max-batch-rays", type=int, default=4_000_000,
                     help="serve: rays per served launch (0 = no cap)")
-    ap.add_argument("--chunk-rays", type=int, default=128, help="serve: rays a wave takes per grab (1 = lane refill)")
+    ap.add_argument("--chunk-rays", type=int, default=1, help="serve: rays a wave takes per grab (1 = lane refill)")
     ap.add_argument("--max-inflight", type=int, default=0, help="serve: launches on the device at once (0 = "
                                                                 "library default)")
-    ap.add_argument("--dispatch", choices=("auto", "persistent", "grid"), default="auto",
-                    help="lane-refilling persistent launch or one lane per ray; auto: grid for the coherent "
-                         "primary rays, persistent for incoherent bounces")
+    ap.add_argument("--dispatch", choices=("auto", "persistent", "range", "grid"), default="auto",
+                    help="persistent: lanes refill from all rays; range: each wave refills from its own "
+                         "contiguous range; grid: one lane per ray; auto: grid for the coherent primary "
+                         "rays, persistent for incoherent bounces")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent dispatch: blocks per CU (0 = library "
                                                                  "default)")
     return ap.parse_args()
@@ -74,16 +75,17 @@ def configure_dispatch(dev, args, auto: str):
 
     if args.dispatch == "auto":
         args.dispatch = auto
-    dev.set_option(B.RTQ_OPTION_PERSISTENT, 1 if args.dispatch == "persistent" else 0)
+    dev.set_option(B.RTQ_OPTION_PERSISTENT, {"grid": 0, "persistent": 1, "range": 2}[args.dispatch])
     if args.blocks_per_cu:
         dev.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, args.blocks_per_cu)
     return {"mode": args.dispatch, "blocks_per_cu": dev.get_option(B.RTQ_OPTION_BLOCKS_PER_CU)}
 
 
 def kernel_name(args, depth: int, any_hit: bool = False) -> str:
-    base = "kTracePersistent" if args.dispatch == "persistent" else ("kTraceAny" if any_hit else "kTraceClosest")
-    flag = ("true" if any_hit else "false") if args.dispatch == "persistent" else "false"
-    return f"{base}<{stack_class(depth)},{flag}>"
+    if args.dispatch in ("persistent", "range"):
+        base = "kTracePersistent" if args.dispatch == "persistent" else "kTraceWaveRange"
+        return f"{base}<{stack_class(depth)},{'true' if any_hit else 'false'}>"
+    return f"{'kTraceAny' if any_hit else 'kTraceClosest'}<{stack_class(depth)},false>"
 
 
 def measured_traffic(workload: str, kernel: str):

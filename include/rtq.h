@@ -106,11 +106,13 @@ const char* rtq_last_error(void);
 rtq_status rtq_device_create(int hip_ordinal, rtq_device** out);
 rtq_status rtq_device_destroy(rtq_device* dev);
 
-/* Dispatch tuning. RTQ_OPTION_PERSISTENT: 1 = rtq_dispatch_rays uses a fixed grid of waves that
- * refill finished lanes with new rays (results identical; faster for incoherent rays such as diffuse
- * bounces), 0 = one lane per ray over a grid of ceil(n/128) blocks (default; faster for coherent
- * primary rays, whose neighbouring lanes share nodes). RTQ_OPTION_BLOCKS_PER_CU: persistent blocks of
- * 128 lanes per compute unit (1..16, default 8). */
+/* Dispatch tuning; results are identical in every mode. RTQ_OPTION_PERSISTENT:
+ *   0 = one lane per ray over a grid of ceil(n/128) blocks (default);
+ *   1 = a fixed grid of waves whose lanes refill with the next unclaimed rays of the whole dispatch
+ *       after each leaf (best for incoherent rays such as diffuse bounces);
+ *   2 = a fixed grid in which each wave owns a contiguous range of rays and refills its lanes from
+ *       it (lanes keep neighbouring, coherent rays).
+ * RTQ_OPTION_BLOCKS_PER_CU: persistent blocks of 128 lanes per compute unit (1..16, default 8). */
 #define RTQ_OPTION_PERSISTENT 1u
 #define RTQ_OPTION_BLOCKS_PER_CU 2u
 rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t value);
@@ -282,7 +284,7 @@ typedef struct rtq_server_config {
   uint32_t max_batch_rays;   /* 0: no cap besides the budget */
   uint32_t plain;            /* 1: one launch per request, no batching (comparison mode) */
   uint32_t max_inflight;     /* batches (or plain launches) on the device at once, 1..8; 0: 3 */
-  rtq_batch_params params;   /* persistent-kernel settings, e.g. from rtq_batch_autotune */
+  rtq_batch_params params;   /* kernel settings; chunk_rays 0 = defaults (lane refill, 8 blocks/CU) */
 } rtq_server_config;
 
 typedef struct rtq_request_stats {

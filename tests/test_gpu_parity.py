@@ -119,7 +119,7 @@ def test_probe_field_tracing_matches_oracle():
         assert hit.any() and (o[hit, 4] != 0).any()
 
 
-@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("persistent", [0, 1, 2])
 @pytest.mark.parametrize("blocks", [1, 8])
 def test_dispatch_modes_identical(persistent, blocks):
     _print(f"case: dispatch persistent={persistent} blocks_per_cu={blocks}, spheres cut + soup bounce rays")

@@ -261,20 +261,63 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s, const float
 }
 
 // ---- persistent traversal with lane refill ------------------------------------------------
-// A fixed grid of waves; each lane traces one ray at a time. After every descent-to-leaf step
-// the lanes whose ray has finished take new rays from a device counter (one atomic per wave), so
-// lanes no longer idle while the slowest ray of their wave finishes. Each ray still follows the
-// reference visit order exactly; only the assignment of rays to lanes changes.
-template <int STACK, bool ANY>
-__global__ __launch_bounds__(kTraceBlock) void kTracePersistent(DevScene s, const float4* __restrict__ rays,
-                                                                uint32_t n, void* __restrict__ out,
-                                                                uint32_t* __restrict__ leafNid,
-                                                                uint32_t* __restrict__ counter) {
-  __shared__ uint32_t lds[STACK * kTraceBlock];
-  uint32_t* stk = lds + threadIdx.x;
+// A fixed grid of waves; each lane traces one ray at a time. After every descent-to-leaf step the
+// lanes whose ray has finished take new rays, so lanes no longer idle while the slowest ray of their
+// wave finishes. Each ray still follows the reference visit order exactly; only the assignment of
+// rays to lanes changes. Two sources of rays:
+//   RANGE = false: chunks of kRefillChunk consecutive rays taken from a device counter when the
+//     wave's current chunk runs out (one atomic per chunk, so waves balance dynamically);
+//   RANGE = true: one fixed contiguous range per wave (no atomics).
+// Either way a wave's lanes hold nearby rays of the buffer.
+constexpr uint32_t kRefillChunk = 256;
+
+// Descend from `node` to the next leaf (or to the end of the traversal) for one lane, testing both
+// child boxes of every inner node as traverse() does. FASTONLY: every active lane of the wave takes
+// the reciprocal slab form (decided per wave), so no per-lane branch is needed.
+template <int STACK, bool FASTONLY>
+__device__ __forceinline__ void descend(const DevNode* __restrict__ nodes, uint32_t* stk, const Ray3& r,
+                                        const RayRcp& rq, bool fast, float lim, uint32_t& node, int& sp) {
+  while (node < kRefDone) {
+    const float4* np = reinterpret_cast<const float4*>(nodes + node);
+    const float4 a = np[0], b = np[1], c = np[2];
+    const uint4 d = reinterpret_cast<const uint4*>(np)[3];
+    float d0, d1;
+    if (FASTONLY || fast) {
+      d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+      d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+    } else {
+      d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+      d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+    }
+    const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
+    if (h0 && h1) {
+      const bool n1 = d1 < d0;
+      if (sp < STACK) stk[sp * kTraceBlock] = n1 ? d.x : d.y;
+      ++sp;
+      node = n1 ? d.y : d.x;
+    } else if (h0 || h1) {
+      node = h0 ? d.x : d.y;
+    } else {
+      node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
+    }
+  }
+}
+
+template <int STACK, bool ANY, bool RANGE>
+__device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, const float4* __restrict__ rays,
+                                            uint32_t n, void* __restrict__ out, uint32_t* __restrict__ leafNid,
+                                            uint32_t* __restrict__ counter) {
   const uint32_t lane = threadIdx.x & 63u;
   bool active = false, fast = false, exhausted = false;
   uint32_t rid = 0, node = kRefDone;
+  uint32_t next = 0, end = 0;  // the wave's current source of rays [next, end)
+  if (RANGE) {
+    const uint32_t waves = gridDim.x * (kTraceBlock / 64u);
+    const uint32_t wave = xcdBlock() * (kTraceBlock / 64u) + (threadIdx.x >> 6);
+    next = uint32_t(uint64_t(n) * wave / waves);
+    end = uint32_t(uint64_t(n) * (wave + 1u) / waves);
+    exhausted = next >= end;
+  }
   int sp = 0;
   float tmax = 0.0f;
   Ray3 r{};
@@ -283,53 +326,43 @@ __global__ __launch_bounds__(kTraceBlock) void kTracePersistent(DevScene s, cons
   for (;;) {
     const uint64_t idle = __ballot(!active);
     if (idle && !exhausted) {
-      const uint32_t want = uint32_t(__popcll(idle));
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(counter, want);
-      base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
-      if (base + want >= n) exhausted = true;
-      if (!active) {
-        const uint32_t k = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
-        if (k < n) {
-          rid = k;
-          r = loadRay(rays, k, tmax);
-          fast = fastLane(s, r, tmax);
-          if (fast) rq = RayRcp{refinedRcp(r.dx), refinedRcp(r.dy), refinedRcp(r.dz)};
-          hit = Closest{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
-          node = s.root;
-          sp = 0;
-          active = true;
+      if (!RANGE && next >= end) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(counter, kRefillChunk);
+        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+        if (base >= n) {
+          exhausted = true;
+        } else {
+          next = base;
+          end = uint32_t(min(uint64_t(n), uint64_t(base) + kRefillChunk));
+        }
+      }
+      if (next < end) {
+        const uint32_t want = uint32_t(__popcll(idle));
+        const uint32_t base = next;
+        next = (end - next > want) ? next + want : end;
+        if (RANGE && next >= end) exhausted = true;
+        if (!active) {
+          const uint32_t k = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+          if (k < next) {
+            rid = k;
+            r = loadRay(rays, k, tmax);
+            fast = fastLane(s, r, tmax);
+            if (fast) rq = RayRcp{refinedRcp(r.dx), refinedRcp(r.dy), refinedRcp(r.dz)};
+            hit = Closest{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
+            node = s.root;
+            sp = 0;
+            active = true;
+          }
         }
       }
     }
     if (!__any(active)) break;
     if (!active) continue;
     bool accepted = false;
-    while (node < kRefDone) {
-      const float4* np = reinterpret_cast<const float4*>(s.nodes + node);
-      const float4 a = np[0], b = np[1], c = np[2];
-      const uint4 d = reinterpret_cast<const uint4*>(np)[3];
-      const float lim = ANY ? tmax : hit.t;
-      float d0, d1;
-      if (fast) {
-        d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
-        d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
-      } else {
-        d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, lim);
-        d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, lim);
-      }
-      const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
-      if (h0 && h1) {
-        const bool n1 = d1 < d0;
-        if (sp < STACK) stk[sp * kTraceBlock] = n1 ? d.x : d.y;
-        ++sp;
-        node = n1 ? d.y : d.x;
-      } else if (h0 || h1) {
-        node = h0 ? d.x : d.y;
-      } else {
-        node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
-      }
-    }
+    const float lim = ANY ? tmax : hit.t;
+    if (__all(fast ? 1 : 0)) descend<STACK, true>(s.nodes, stk, r, rq, fast, lim, node, sp);
+    else descend<STACK, false>(s.nodes, stk, r, rq, fast, lim, node, sp);
     if (node != kRefDone) {
       if (node != kRefEmpty) {
         uint32_t slot = node & ~kRefLeaf;
@@ -365,6 +398,23 @@ __global__ __launch_bounds__(kTraceBlock) void kTracePersistent(DevScene s, cons
       active = false;
     }
   }
+}
+
+template <int STACK, bool ANY>
+__global__ __launch_bounds__(kTraceBlock) void kTracePersistent(DevScene s, const float4* __restrict__ rays,
+                                                                uint32_t n, void* __restrict__ out,
+                                                                uint32_t* __restrict__ leafNid,
+                                                                uint32_t* __restrict__ counter) {
+  __shared__ uint32_t lds[STACK * kTraceBlock];
+  refillTrace<STACK, ANY, false>(s, lds + threadIdx.x, rays, n, out, leafNid, counter);
+}
+
+template <int STACK, bool ANY>
+__global__ __launch_bounds__(kTraceBlock) void kTraceWaveRange(DevScene s, const float4* __restrict__ rays,
+                                                               uint32_t n, void* __restrict__ out,
+                                                               uint32_t* __restrict__ leafNid) {
+  __shared__ uint32_t lds[STACK * kTraceBlock];
+  refillTrace<STACK, ANY, true>(s, lds + threadIdx.x, rays, n, out, leafNid, nullptr);
 }
 
 // ---- camera rays generated in the kernel --------------------------------------------------
@@ -683,11 +733,12 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* 
   }
 }
 
-// Lane-refilling form of the batch: lanes take single rays from the concatenation of all items
-// (one atomic per wave, a binary search over rayBegin per lane), so a lane that finishes early
-// picks up a ray of any item, scene or mode instead of idling until its wave's slowest ray is done.
-// The scene pointers and the closest/any mode are per lane. Each ray still follows the reference
-// visit order exactly; every wave reaches the exit test once the counter passes totalRays.
+// Lane-refilling form of the batch: lanes take single rays from the concatenation of all items (the
+// wave draws chunks of kRefillChunk rays from a device counter; a binary search over rayBegin maps a
+// ray to its item), so a lane that finishes early picks up the next ray, whatever its item, scene or
+// mode, instead of idling until its wave's slowest ray is done. The scene pointers and the
+// closest/any mode are per lane. Each ray still follows the reference visit order exactly; every wave
+// reaches the exit test once the counter passes totalRays.
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __restrict__ scenes,
                                                             const BatchItemDev* __restrict__ items,
@@ -698,6 +749,7 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
   const uint32_t lane = threadIdx.x & 63u;
   bool active = false, fast = false, exhausted = false, anyRay = false;
   uint32_t rid = 0, item = 0, node = kRefDone;
+  uint32_t next = 0, end = 0;  // the wave's current chunk of rays [next, end)
   int sp = 0;
   float tmax = 0.0f;
   const DevNode* nodes = nullptr;
@@ -707,15 +759,24 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
   Closest hit{};
   for (;;) {
     const uint64_t idle = __ballot(!active);
-    if (idle && !exhausted) {
-      const uint32_t want = uint32_t(__popcll(idle));
+    if (idle && !exhausted && next >= end) {
       uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(counter, want);
+      if (lane == 0) base = atomicAdd(counter, kRefillChunk);
       base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
-      if (uint64_t(base) + want >= totalRays) exhausted = true;
+      if (base >= totalRays) {
+        exhausted = true;
+      } else {
+        next = base;
+        end = uint32_t(min(uint64_t(totalRays), uint64_t(base) + kRefillChunk));
+      }
+    }
+    if (idle && next < end) {
+      const uint32_t want = uint32_t(__popcll(idle));
+      const uint32_t base = next;
+      next = (end - next > want) ? next + want : end;
       if (!active) {
         const uint32_t k = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
-        if (k < totalRays) {
+        if (k < next) {
           uint32_t lo = 0, hi = numItems;
           while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;
@@ -742,30 +803,8 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
     if (!__any(active)) break;
     if (!active) continue;
     bool accepted = false;
-    while (node < kRefDone) {
-      const float4* np = reinterpret_cast<const float4*>(nodes + node);
-      const float4 a = np[0], b = np[1], c = np[2];
-      const uint4 d = reinterpret_cast<const uint4*>(np)[3];
-      float d0, d1;
-      if (fast) {
-        d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, hit.t);
-        d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, hit.t);
-      } else {
-        d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, hit.t);
-        d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, hit.t);
-      }
-      const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
-      if (h0 && h1) {
-        const bool n1 = d1 < d0;
-        if (sp < STACK) stk[sp * kTraceBlock] = n1 ? d.x : d.y;
-        ++sp;
-        node = n1 ? d.y : d.x;
-      } else if (h0 || h1) {
-        node = h0 ? d.x : d.y;
-      } else {
-        node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
-      }
-    }
+    if (__all(fast ? 1 : 0)) descend<STACK, true>(nodes, stk, r, rq, fast, hit.t, node, sp);
+    else descend<STACK, false>(nodes, stk, r, rq, fast, hit.t, node, sp);
     if (node != kRefDone) {
       if (node != kRefEmpty) {
         uint32_t slot = node & ~kRefLeaf;

@@ -152,7 +152,7 @@ rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t valu
   if (!dev) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL device");
   switch (option) {
     case RTQ_OPTION_PERSISTENT:
-      if (value > 1) return fail(RTQ_ERR_INVALID_ARGUMENT, "persistent is 0 or 1");
+      if (value > 2) return fail(RTQ_ERR_INVALID_ARGUMENT, "persistent is 0, 1 or 2");
       dev->persistent = value;
       return RTQ_OK;
     case RTQ_OPTION_BLOCKS_PER_CU:
@@ -472,6 +472,19 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(rays);
+  if (!counters && s->dev->persistent == 2) {
+    const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
+    const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, n + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
+    withStack(s->info.max_depth, [&](auto S) {
+      constexpr int K = decltype(S)::value;
+      if (flags == RTQ_TRACE_CLOSEST)
+        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid);
+      else
+        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, true>), pgrid, block, 0, st, s->view, r, n, out, nullptr);
+    });
+    RTQ_HIP(hipGetLastError());
+    return RTQ_OK;
+  }
   if (!counters && s->dev->persistent) {
     uint32_t* counter = s->counters + (s->nextCounter.fetch_add(1) % kCounterRing);
     hipLaunchKernelGGL(rtq::kResetCounter, dim3(1), dim3(64), 0, st, counter);

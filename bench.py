@@ -244,6 +244,7 @@ def main():
             "bvh_nodes": int(info.num_nodes),
             "leaf_triangle_slots": int(info.num_leaf_slots),
             "bvh_max_depth": int(info.max_depth),
+            "scene_device_bytes": int(info.device_bytes),
         },
         "roofline": {
             "bound": "hbm",

@@ -1,0 +1,178 @@
+// CPU test of upload-BVH's host half (tim_raytracing_amd/csrc/device/scene_layout.h): the device
+// layout built from builder blobs keeps every reference node, box, child link, leaf and triangle, lays
+// sibling inner nodes out as aligned pairs in depth-first order, and rejects broken blobs. Prints "ok".
+#include <cstdio>
+#include <cstring>
+#include <set>
+#include <vector>
+
+#include "../../include/rtq_build.h"
+#include "../../tim_raytracing_amd/csrc/device/scene_layout.h"
+
+namespace {
+
+int g_fail = 0;
+#define CHECK(c)                                                     \
+  do {                                                               \
+    if (!(c)) {                                                      \
+      std::fprintf(stderr, "%s:%d: CHECK(%s)\n", __FILE__, __LINE__, #c); \
+      ++g_fail;                                                      \
+      return;                                                        \
+    }                                                                \
+  } while (0)
+
+struct Built {
+  std::vector<uint8_t> blob;
+  rtq_ranges ranges{};
+  std::vector<float> pos;
+};
+
+Built build(uint32_t kind, uint64_t seed, uint32_t param, const std::vector<float>* tri = nullptr) {
+  rtq_builder* b = nullptr;
+  rtq_builder_create(&b);
+  if (tri) {
+    uint32_t off = 0;
+    rtq_builder_add_vertices(b, tri->data(), uint32_t(tri->size() / 3), &off);
+    std::vector<uint32_t> idx(tri->size() / 3);
+    for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    rtq_builder_add_triangle_list(b, off, uint32_t(idx.size() / 3), idx.data(), nullptr);
+  } else {
+    rtq_scene_generate(b, kind, seed, param);
+  }
+  rtq_builder_build(b, 2);
+  Built out;
+  size_t bytes = 0;
+  rtq_builder_gpu_size(b, &bytes);
+  out.blob.resize(bytes);
+  rtq_builder_fill_gpu_buffer(b, out.blob.data(), bytes, &out.ranges);
+  const float* xyz = nullptr;
+  uint32_t n = 0;
+  rtq_builder_positions(b, &xyz, &n);
+  out.pos.assign(xyz, xyz + 3 * size_t(n));
+  rtq_builder_destroy(b);
+  return out;
+}
+
+void checkScene(const Built& s) {
+  rtq::HostScene h;
+  const char* why = "";
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), uint32_t(s.pos.size() / 3), h,
+                            &why) == RTQ_OK);
+  const uint32_t* nodes = reinterpret_cast<const uint32_t*>(s.blob.data() + s.ranges.nodes.offset);
+  const uint32_t* leaf = reinterpret_cast<const uint32_t*>(s.blob.data() + s.ranges.leaf_data.offset);
+  const uint32_t* tris = reinterpret_cast<const uint32_t*>(s.blob.data() + s.ranges.triangles.offset);
+  const uint32_t numNodes = s.ranges.nodes.size / 64;
+  CHECK(h.numNodes == numNodes && h.nodes.size() % 2 == 0);
+  std::set<uint32_t> refNodes, usedSlots, innerSlots;
+  std::vector<uint32_t> pairOrder;  // pair bases in depth-first (child 0 first) order
+  uint32_t inner = 0, leaves = 0;
+  // walk device refs and reference links side by side
+  struct Item { uint32_t ref, id; };
+  std::vector<Item> st;
+  if (numNodes == 1) {
+    st.push_back({h.root, 0});
+  } else {
+    CHECK(h.root == 0);
+    st.push_back({0u, 0u});
+  }
+  while (!st.empty()) {
+    const Item it = st.back();
+    st.pop_back();
+    CHECK(refNodes.insert(it.id).second);
+    if (it.ref == rtq::kRefEmpty || (it.ref & rtq::kRefLeaf)) {
+      ++leaves;
+      const uint32_t off = nodes[16 * it.id + 3];
+      const uint32_t count = off == 0xFFFFFFFFu ? 0u : (leaf[off] & 0xFFFu);
+      if (count == 0) {
+        CHECK(it.ref == rtq::kRefEmpty);
+        continue;
+      }
+      CHECK(it.ref != rtq::kRefEmpty);
+      const uint32_t first = it.ref & ~rtq::kRefLeaf;
+      for (uint32_t k = 0; k < count; ++k) {
+        CHECK(first + k < h.tris.size());
+        CHECK(usedSlots.insert(first + k).second);
+        const rtq::DevTri& d = h.tris[first + k];
+        uint32_t w[4];
+        std::memcpy(w, d.t2, 16);
+        CHECK(w[2] == it.id && w[3] == (k + 1 == count ? 1u : 0u));
+        const uint32_t* t = leaf + off + 1 + 3 * k;
+        CHECK(w[1] < s.ranges.triangles.size / 12);
+        CHECK(std::memcmp(tris + 3 * w[1], t, 12) == 0);
+        const uint32_t v0 = t[0] + (t[1] & 0xFFFFu);
+        CHECK(std::memcmp(d.t0, &s.pos[3 * size_t(v0)], 12) == 0);
+      }
+      continue;
+    }
+    ++inner;
+    CHECK(it.ref < h.nodes.size());
+    CHECK(innerSlots.insert(it.ref).second);
+    const rtq::DevNode& d = h.nodes[it.ref];
+    CHECK(d.d[2] == it.id);
+    const float* f = reinterpret_cast<const float*>(nodes + 16 * it.id);
+    const float want[12] = {f[4], f[5], f[12], f[6], f[7], f[13], f[8], f[9], f[14], f[10], f[11], f[15]};
+    const float got[12] = {d.a[0], d.a[1], d.a[2], d.a[3], d.b[0], d.b[1], d.b[2], d.b[3], d.c[0], d.c[1], d.c[2], d.c[3]};
+    CHECK(std::memcmp(want, got, sizeof(want)) == 0);
+    const uint32_t packed = nodes[16 * it.id + 2];
+    const uint32_t l = packed & 0x3FFFFFFFu;
+    const bool leafChild[2] = {(packed & 0x80000000u) != 0, (packed & 0x40000000u) != 0};
+    uint32_t base = 0xFFFFFFFFu;
+    for (int k = 0; k < 2; ++k) {
+      if (leafChild[k]) {
+        CHECK(d.d[k] == rtq::kRefEmpty || (d.d[k] & rtq::kRefLeaf));
+        continue;
+      }
+      CHECK(d.d[k] < h.nodes.size() && (d.d[k] & 1u) == uint32_t(k));  // child k in half k of its pair
+      const uint32_t b = d.d[k] - uint32_t(k);
+      CHECK(base == 0xFFFFFFFFu || base == b);                         // both inner children share the pair
+      base = b;
+    }
+    if (base != 0xFFFFFFFFu) pairOrder.push_back(base);
+    for (int k = 1; k >= 0; --k) st.push_back({d.d[k], l + uint32_t(k)});
+  }
+  CHECK(inner == h.numInner);
+  CHECK(usedSlots.size() == h.tris.size());
+  // depth-first layout: pairs appear in walk order right after the root's pair
+  for (size_t i = 0; i < pairOrder.size(); ++i) CHECK(pairOrder[i] == 2 + 2 * i);
+  CHECK(h.nodes.size() == (numNodes == 1 ? 0u : 2 + 2 * pairOrder.size()));
+  // every slot that holds no reachable inner node is an all-zero record
+  static const rtq::DevNode zero{};
+  for (size_t k = 0; k < h.nodes.size(); ++k)
+    if (!innerSlots.count(uint32_t(k))) CHECK(std::memcmp(&h.nodes[k], &zero, sizeof(zero)) == 0);
+}
+
+void checkRejects(Built s) {
+  rtq::HostScene h;
+  const char* why = "";
+  rtq_ranges bad = s.ranges;
+  bad.nodes.offset = uint32_t(s.blob.size());  // outside the blob
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &bad, s.pos.data(), uint32_t(s.pos.size() / 3), h, &why) ==
+        RTQ_ERR_BAD_BLOB);
+  // a child id beyond the node count
+  uint32_t* nodes = reinterpret_cast<uint32_t*>(s.blob.data() + s.ranges.nodes.offset);
+  const uint32_t keep = nodes[2];
+  nodes[2] = (keep & 0xC0000000u) | 0x3FFFFFF0u;
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), uint32_t(s.pos.size() / 3), h,
+                            &why) == RTQ_ERR_BAD_BLOB);
+  nodes[2] = keep;
+  // too few positions for the triangles
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), 2, h, &why) == RTQ_ERR_BAD_BLOB);
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), uint32_t(s.pos.size() / 3), h,
+                            &why) == RTQ_OK);
+}
+
+}  // namespace
+
+int main() {
+  checkScene(build(RTQ_SCENE_BOX, 1, 0));
+  checkScene(build(RTQ_SCENE_SPHERES, 2, 64));
+  checkScene(build(RTQ_SCENE_SOUP, 3, 3000));
+  const std::vector<float> one = {0, 0, 0, 1, 0, 0, 0, 1, 0};
+  checkScene(build(0, 0, 0, &one));  // a single leaf as the root
+  const std::vector<float> two = {0, 0, 0, 1, 0, 0, 0, 1, 0, 5, 5, 5, 6, 5, 5, 5, 6, 5};
+  checkScene(build(0, 0, 0, &two));
+  checkRejects(build(RTQ_SCENE_SPHERES, 4, 8));
+  if (g_fail) return 1;
+  std::puts("ok");
+  return 0;
+}

@@ -1,0 +1,17 @@
+"""CPU test of upload-BVH's host half: blob decode and device layout (scene_layout.h), no GPU needed."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+def test_scene_layout(tmp_path):
+    exe = tmp_path / "scene_layout_test"
+    pkg = os.path.join(ROOT, "tim_raytracing_amd")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-Wall", "-Xlinker", "--no-as-needed",
+                    "-fsanitize=address,undefined", "-o", str(exe), os.path.join(HERE, "cpp", "scene_layout_test.cpp"),
+                    os.path.join(pkg, "librtq_build.so"), f"-Wl,-rpath,{pkg}"], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert r.stdout.strip() == "ok"

@@ -1,6 +1,8 @@
 // Device-resident scene layout shared by the upload code and the kernels.
 //
-// Inner nodes: one 64 B record per inner node of the reference tree (compact, reference order):
+// Inner nodes: one 64 B record per inner node of the reference tree, laid out depth first with the
+// two children of a node side by side in one 128 B pair (the root alone in pair 0; the half of a pair
+// whose child is a leaf stays empty):
 //   a = (b0.min.x, b0.min.y, b0.min.z, b0.max.x)
 //   b = (b0.max.y, b0.max.z, b1.min.x, b1.min.y)
 //   c = (b1.min.z, b1.max.x, b1.max.y, b1.max.z)
@@ -8,7 +10,7 @@
 // A child ref is an inner index (< kRefDone), kRefLeaf | first triangle slot of a leaf, or kRefEmpty
 // for a leaf holding no triangle. Leaves need no record: visiting one reads its triangle slots.
 //
-// Triangle slots: one 48 B record per inline triangle of every leaf, leaves in reference node
+// Triangle slots: one 48 B record per inline triangle of every leaf, leaves in the same depth-first
 // order, triangles in blob (sorted leaf) order:
 //   t0 = (p0.x, p0.y, p0.z, e1.x), t1 = (e1.y, e1.z, e2.x, e2.y),
 //   t2 = (e2.z, triangle id bits, leaf reference node id bits, 1 if last slot of its leaf else 0)

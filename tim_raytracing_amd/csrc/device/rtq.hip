@@ -6,12 +6,14 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
 
 #include "../../../include/rtq.h"
+#include "scene_layout.h"
 #include "kernels.h"
 #include "layout.h"
 
@@ -54,20 +56,6 @@ rtq_status hipFail(hipError_t e, const char* what) {
     hipError_t e_ = (call);                        \
     if (e_ != hipSuccess) return hipFail(e_, #call); \
   } while (0)
-
-struct TriKey {
-  uint32_t a, b, c;
-  bool operator==(const TriKey& o) const { return a == o.a && b == o.b && c == o.c; }
-};
-struct TriKeyHash {
-  size_t operator()(const TriKey& k) const {
-    uint64_t h = 1469598103934665603ull;
-    h = (h ^ k.a) * 1099511628211ull;
-    h = (h ^ k.b) * 1099511628211ull;
-    h = (h ^ k.c) * 1099511628211ull;
-    return size_t(h ^ (h >> 32));
-  }
-};
 
 template <class T>
 rtq_status uploadArray(rtq_scene* s, const std::vector<T>& host, const T** dptr) {
@@ -279,174 +267,53 @@ rtq_status rtq_read_hits(rtq_device* dev, const void* d_src, void* h_dst, size_t
   return RTQ_OK;
 }
 
-// upload-BVH: decode the fillGpuBuffer blob (BVHBuilder.cpp:1319-1534) as the shaders bind it
-// (BVHData::fillBvhBindings, BVHData.cpp:17-26) and rebuild it in the device layout.
+// upload-BVH: decode the blob and build the device layout on the host (scene_layout.h), then copy it
+// to HBM where it stays resident.
 rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, const rtq_ranges* ranges,
                             const float* positions, uint32_t numVertices, rtq_scene** out) {
   if (!dev || !blobPtr || !ranges || !out || (!positions && numVertices))
     return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
-  const uint8_t* blob = static_cast<const uint8_t*>(blobPtr);
-  auto inBlob = [&](const rtq_range& r) { return size_t(r.offset) + r.size <= bytes && r.offset % 4 == 0; };
-  if (!inBlob(ranges->triangles) || !inBlob(ranges->lights) || !inBlob(ranges->nodes) || !inBlob(ranges->leaf_data))
-    return fail(RTQ_ERR_BAD_BLOB, "a range points outside the blob");
-  const uint32_t numNodes = ranges->nodes.size / 64u;
-  const uint32_t leafWords = ranges->leaf_data.size / 4u;
-  const uint32_t numTris = ranges->triangles.size / 12u;
-  const uint32_t numLights = ranges->lights.size / 56u;
-  if (numNodes == 0) return fail(RTQ_ERR_BAD_BLOB, "no nodes");
-  const uint32_t* nodes = reinterpret_cast<const uint32_t*>(blob + ranges->nodes.offset);
-  const uint32_t* leaf = reinterpret_cast<const uint32_t*>(blob + ranges->leaf_data.offset);
-  const uint32_t* tris = reinterpret_cast<const uint32_t*>(blob + ranges->triangles.offset);
-  const uint32_t* lightsRaw = reinterpret_cast<const uint32_t*>(blob + ranges->lights.offset);
+  rtq::HostScene h;
+  const char* why = "";
+  rtq_status st;
+  try {
+    st = rtq::buildHostScene(static_cast<const uint8_t*>(blobPtr), bytes, ranges, positions, numVertices, h, &why);
+  } catch (const std::bad_alloc&) {
+    return fail(RTQ_ERR_OUT_OF_MEMORY, "host allocation failed");
+  }
+  if (st != RTQ_OK) return fail(st, why);
 
   rtq_scene* s = new rtq_scene();
   s->dev = dev;
-  auto bail = [&](rtq_status st) {
+  auto bail = [&](rtq_status e) {
     freeScene(s);
-    return st;
+    return e;
   };
-
-  // triangle triple -> first triangle index (the hit record's tri_id)
-  std::unordered_map<TriKey, uint32_t, TriKeyHash> triIndex;
-  triIndex.reserve(size_t(numTris) * 2);
-  for (uint32_t i = 0; i < numTris; ++i) triIndex.emplace(TriKey{tris[3 * i], tris[3 * i + 1], tris[3 * i + 2]}, i);
-
-  // Walk the tree from the root through child pointers (bvhGetter.glsl:17-25); numbering the
-  // inner nodes in reference order keeps each sibling pair's parent record in one place.
-  std::vector<uint32_t> innerIndex(numNodes, 0xFFFFFFFFu);
-  std::vector<uint8_t> isLeaf(numNodes, 0), seen(numNodes, 0);
-  std::vector<uint32_t> leafSlot(numNodes, rtq::kRefEmpty);
-  const bool rootIsLeaf = numNodes == 1;
-  uint32_t maxDepth = 0;
-  {
-    struct Item { uint32_t id; uint32_t depth; bool leaf; };
-    std::vector<Item> st{{0u, 0u, rootIsLeaf}};
-    while (!st.empty()) {
-      Item it = st.back();
-      st.pop_back();
-      if (it.id >= numNodes) return bail(fail(RTQ_ERR_BAD_BLOB, "child id beyond node count"));
-      if (seen[it.id]) return bail(fail(RTQ_ERR_BAD_BLOB, "node reached twice"));
-      seen[it.id] = 1;
-      if (it.depth > 64) return bail(fail(RTQ_ERR_LIMIT, "tree deeper than 64 levels"));
-      if (it.leaf) {
-        isLeaf[it.id] = 1;
-        maxDepth = std::max(maxDepth, it.depth);
-        continue;
-      }
-      const uint32_t packed = nodes[16 * it.id + 2];
-      const uint32_t l = packed & 0x3FFFFFFFu;
-      if (l + 1u >= numNodes) return bail(fail(RTQ_ERR_BAD_BLOB, "child id beyond node count"));
-      st.push_back({l + 1u, it.depth + 1u, (packed & 0x40000000u) != 0});
-      st.push_back({l, it.depth + 1u, (packed & 0x80000000u) != 0});
-    }
-  }
-  std::vector<rtq::DevTri> dtris;
-  std::vector<uint32_t> innerIds;
-  for (uint32_t id = 0; id < numNodes; ++id) {
-    if (!seen[id]) continue;
-    if (!isLeaf[id]) {
-      innerIndex[id] = uint32_t(innerIds.size());
-      innerIds.push_back(id);
-      continue;
-    }
-    const uint32_t off = nodes[16 * id + 3];
-    if (off == 0xFFFFFFFFu) continue;
-    if (off >= leafWords) return bail(fail(RTQ_ERR_BAD_BLOB, "leaf data offset beyond the leaf range"));
-    const uint32_t count = leaf[off] & 0xFFFu;
-    if (count == 0) continue;
-    if (size_t(off) + 1 + 3 * size_t(count) > leafWords) return bail(fail(RTQ_ERR_BAD_BLOB, "leaf triangles overrun"));
-    leafSlot[id] = rtq::kRefLeaf | uint32_t(dtris.size());
-    for (uint32_t k = 0; k < count; ++k) {
-      const uint32_t* t = leaf + off + 1 + 3 * k;
-      uint32_t vi[3] = {t[0] + (t[1] & 0xFFFFu), t[0] + (t[1] >> 16), t[0] + (t[2] & 0xFFFFu)};
-      float p[3][3];
-      for (int q = 0; q < 3; ++q) {
-        if (vi[q] >= numVertices) return bail(fail(RTQ_ERR_BAD_BLOB, "triangle vertex beyond position count"));
-        p[q][0] = positions[3 * size_t(vi[q])];
-        p[q][1] = positions[3 * size_t(vi[q]) + 1];
-        p[q][2] = positions[3 * size_t(vi[q]) + 2];
-      }
-      auto found = triIndex.find(TriKey{t[0], t[1], t[2]});
-      const uint32_t triId = found == triIndex.end() ? RTQ_NO_HIT : found->second;
-      rtq::DevTri d{};
-      const float e1[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
-      const float e2[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
-      d.t0[0] = p[0][0]; d.t0[1] = p[0][1]; d.t0[2] = p[0][2]; d.t0[3] = e1[0];
-      d.t1[0] = e1[1]; d.t1[1] = e1[2]; d.t1[2] = e2[0]; d.t1[3] = e2[1];
-      uint32_t w[4] = {0, triId, id, k + 1 == count ? 1u : 0u};
-      std::memcpy(&d.t2, w, sizeof(w));
-      d.t2[0] = e2[2];
-      dtris.push_back(d);
-    }
-  }
-  auto ref = [&](uint32_t id) { return isLeaf[id] ? leafSlot[id] : innerIndex[id]; };
-  std::vector<rtq::DevNode> dnodes(innerIds.size());
-  for (size_t k = 0; k < innerIds.size(); ++k) {
-    const uint32_t id = innerIds[k];
-    const uint32_t* n = nodes + 16 * id;
-    const float* f = reinterpret_cast<const float*>(n);
-    const float* n0xy = f + 4;
-    const float* n1xy = f + 8;
-    const float* nz = f + 12;
-    rtq::DevNode& d = dnodes[k];
-    d.a[0] = n0xy[0]; d.a[1] = n0xy[1]; d.a[2] = nz[0]; d.a[3] = n0xy[2];
-    d.b[0] = n0xy[3]; d.b[1] = nz[1]; d.b[2] = n1xy[0]; d.b[3] = n1xy[1];
-    d.c[0] = nz[2]; d.c[1] = n1xy[2]; d.c[2] = n1xy[3]; d.c[3] = nz[3];
-    const uint32_t l = n[2] & 0x3FFFFFFFu;
-    d.d[0] = ref(l);
-    d.d[1] = ref(l + 1);
-    d.d[2] = id;
-    d.d[3] = 0;
-  }
-  // the fast slab division needs every box bound to be 0 or of magnitude in [2^-37, 2^59]
-  bool fastBoxes = true;
-  for (const rtq::DevNode& d : dnodes) {
-    for (int k = 0; k < 4; ++k) {
-      const float v[3] = {d.a[k], d.b[k], d.c[k]};
-      for (float x : v) {
-        const float ax = std::fabs(x);
-        if (!(ax == 0.0f || (ax >= 0x1p-37f && ax <= 0x1p59f))) fastBoxes = false;
-      }
-    }
-  }
-  std::vector<uint32_t> leafOff(numNodes);
-  for (uint32_t id = 0; id < numNodes; ++id) leafOff[id] = nodes[16 * id + 3];
-  std::vector<uint32_t> leafData(leaf, leaf + leafWords);
-  std::vector<float> lights(size_t(numLights) * 4);
-  std::vector<uint32_t> kinds(numLights);
-  for (uint32_t i = 0; i < numLights; ++i) {
-    const uint32_t* L = lightsRaw + 14 * size_t(i);
-    kinds[i] = L[0];
-    std::memcpy(&lights[4 * size_t(i)], L + 1, 16);
-  }
-
   hipError_t e = hipSetDevice(dev->ordinal);
   if (e != hipSuccess) return bail(hipFail(e, "hipSetDevice"));
-  rtq_status st;
-  if ((st = uploadArray(s, dnodes, &s->view.nodes)) != RTQ_OK) return bail(st);
-  if ((st = uploadArray(s, dtris, &s->view.tris)) != RTQ_OK) return bail(st);
-  if ((st = uploadArray(s, leafOff, &s->view.leafOff)) != RTQ_OK) return bail(st);
-  if ((st = uploadArray(s, leafData, &s->view.leafData)) != RTQ_OK) return bail(st);
-  if ((st = uploadArray(s, lights, &s->view.lights)) != RTQ_OK) return bail(st);
-  if ((st = uploadArray(s, kinds, &s->view.lightKind)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, h.nodes, &s->view.nodes)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, h.tris, &s->view.tris)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, h.leafOff, &s->view.leafOff)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, h.leafData, &s->view.leafData)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, h.lights, &s->view.lights)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, h.lightKinds, &s->view.lightKind)) != RTQ_OK) return bail(st);
+  if ((st = uploadArray(s, h.triangles, &s->view.triangles)) != RTQ_OK) return bail(st);
   {
     const uint32_t* dc = nullptr;
     if ((st = uploadArray(s, std::vector<uint32_t>(kCounterRing, 0u), &dc)) != RTQ_OK) return bail(st);
     s->counters = const_cast<uint32_t*>(dc);
   }
-  if ((st = uploadArray(s, std::vector<uint32_t>(tris, tris + 3 * size_t(numTris)), &s->view.triangles)) != RTQ_OK)
-    return bail(st);
-  s->view.root = rootIsLeaf ? leafSlot[0] : 0u;
-  s->view.numNodes = numNodes;
-  s->view.leafWords = leafWords;
-  s->view.numLights = numLights;
-  s->view.fastBoxes = fastBoxes ? 1u : 0u;
-  s->info.num_nodes = numNodes;
-  s->info.num_inner = uint32_t(innerIds.size());
-  s->info.num_leaf_slots = uint32_t(dtris.size());
-  s->info.num_triangles = numTris;
-  s->info.num_lights = numLights;
-  s->info.max_depth = maxDepth;
+  s->view.root = h.root;
+  s->view.numNodes = h.numNodes;
+  s->view.leafWords = h.leafWords;
+  s->view.numLights = h.numLights;
+  s->view.fastBoxes = h.fastBoxes ? 1u : 0u;
+  s->info.num_nodes = h.numNodes;
+  s->info.num_inner = h.numInner;
+  s->info.num_leaf_slots = uint32_t(h.tris.size());
+  s->info.num_triangles = h.numTris;
+  s->info.num_lights = h.numLights;
+  s->info.max_depth = h.maxDepth;
   *out = s;
   return RTQ_OK;
 }

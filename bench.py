@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--chunk-rays", type=int, default=1, help="serve: rays a wave takes per grab (1 = lane refill)")
     ap.add_argument("--max-inflight", type=int, default=0, help="serve: launches on the device at once (0 = "
                                                                 "library default)")
+    ap.add_argument("--ray-order", choices=("row", "tile"), default="tile",
+                    help="primary: order of the ray buffer (tile: 8x8 pixel tiles, one per wave, as a 2D "
+                         "compute dispatch of the camera pass covers the frame; row: y*W + x)")
     ap.add_argument("--dispatch", choices=("auto", "persistent", "range", "grid"), default="auto",
                     help="persistent: lanes refill from all rays; range: each wave refills from its own "
                          "contiguous range; grid: one lane per ray; auto: grid for the coherent primary "
@@ -63,6 +66,15 @@ def parse():
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent dispatch: blocks per CU (0 = library "
                                                                  "default)")
     return ap.parse_args()
+
+
+def tile_order(width: int, height: int, t: int = 8) -> np.ndarray:
+    """Pixel indices y*width + x listed tile by tile (t x t tiles, tiles row-major, pixels row-major
+    inside a tile): one wave of 64 lanes then traces one 8x8 tile, as a 2D compute dispatch does."""
+    assert width % t == 0 and height % t == 0
+    y, x = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
+    idx = (y * width + x).reshape(height // t, t, width // t, t).transpose(0, 2, 1, 3)
+    return np.ascontiguousarray(idx.reshape(-1))
 
 
 def stack_class(depth: int) -> int:
@@ -171,7 +183,10 @@ def main():
     build_s = builder.stats().build_seconds
     blob, ranges = builder.fill_gpu_buffer()
     pos = builder.positions()
-    rays = H.frustum_rays(frustum, W, Hh, seed=2)
+    rays_row = H.frustum_rays(frustum, W, Hh, seed=2)
+    other_order = "row" if args.ray_order == "tile" else "tile"
+    rays_tile = np.ascontiguousarray(rays_row[tile_order(W, Hh)])
+    rays, rays_other = (rays_tile, rays_row) if args.ray_order == "tile" else (rays_row, rays_tile)
     n = rays.size
     setup_s = time.perf_counter() - t0
 
@@ -180,7 +195,7 @@ def main():
     scene = D.Scene(dev, blob, ranges, pos)
     info = scene.info()
     kname = kernel_name(args, int(info.max_depth))
-    traffic, traffic_src = measured_traffic("spheres-1M-primary-1920x1080", kname)
+    traffic, traffic_src = measured_traffic("spheres-1M-primary-1920x1080", f"{kname}@{args.ray_order}")
     stream = dev.stream()
     sptr = stream.ptr
     d_rays = dev.upload(rays)
@@ -222,6 +237,15 @@ def main():
                       max(1, args.warmup // 2), args.steps)
     masks = d_masks.read(np.uint32, n)
 
+    # the camera pass form: rays made in the kernel from the frustum (no jitter), 8x8 pixel tiles per wave
+    import ctypes as C
+    camera_ms = timed(lambda: B.check_device("rtq_trace_camera", dev.lib.rtq_trace_camera(
+        scene.handle, C.byref(frustum), W, Hh, 0, d_hits.ptr, d_nid.ptr, sptr)), max(1, args.warmup // 2), args.steps)
+
+    # the same rays in the other buffer order, for comparison (same kernel, same per-ray work)
+    d_rays.write(rays_other)
+    other_ms = timed(closest, max(1, args.warmup // 2), args.steps)
+
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     result = {
         "metric": "closest-hit Mrays/s @1M tris (frac of HBM roofline)",
@@ -243,6 +267,7 @@ def main():
             "tmax": 100.0,
             "bvh_nodes": int(info.num_nodes),
             "leaf_triangle_slots": int(info.num_leaf_slots),
+            "ray_order": args.ray_order,
             "bvh_max_depth": int(info.max_depth),
             "scene_device_bytes": int(info.device_bytes),
         },
@@ -270,6 +295,10 @@ def main():
             "anyhit_mrays_s": round(n_hit / (shadow_ms * 1e-3) / 1e6, 2),
             "occluded_frac": round(float(((masks >> 1) & 1)[hit_mask].mean()) if n_hit else 0.0, 4),
         },
+        "camera_pass": {"kernel": "kTraceCamera", "ms_per_step": round(camera_ms, 4),
+                        "mrays_s": round(n / (camera_ms * 1e-3) / 1e6, 2)},
+        "other_ray_order": {"ray_order": other_order, "ms_per_step": round(other_ms, 4),
+                            "mrays_s": round(n / (other_ms * 1e-3) / 1e6, 2)},
         "host": {"threads": threads, "bvh_build_s": round(build_s, 2), "setup_s": round(setup_s, 2)},
     }
     if not args.no_cpu_baseline:

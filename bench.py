@@ -63,6 +63,8 @@ def parse():
                     help="persistent: lanes refill from all rays; range: each wave refills from its own "
                          "contiguous range; grid: one lane per ray; auto: grid for the coherent primary "
                          "rays, persistent for incoherent bounces")
+    ap.add_argument("--sort", type=int, choices=(0, 1), default=0,
+                    help="primary/incoherent: sort rays by direction octant and origin cell before tracing")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent dispatch: blocks per CU (0 = library "
                                                                  "default)")
     return ap.parse_args()
@@ -88,9 +90,10 @@ def configure_dispatch(dev, args, auto: str):
     if args.dispatch == "auto":
         args.dispatch = auto
     dev.set_option(B.RTQ_OPTION_PERSISTENT, {"grid": 0, "persistent": 1, "range": 2}[args.dispatch])
+    dev.set_option(B.RTQ_OPTION_SORT_RAYS, args.sort)
     if args.blocks_per_cu:
         dev.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, args.blocks_per_cu)
-    return {"mode": args.dispatch, "blocks_per_cu": dev.get_option(B.RTQ_OPTION_BLOCKS_PER_CU)}
+    return {"mode": args.dispatch, "blocks_per_cu": dev.get_option(B.RTQ_OPTION_BLOCKS_PER_CU), "sort_rays": args.sort}
 
 
 def kernel_name(args, depth: int, any_hit: bool = False) -> str:

@@ -121,12 +121,15 @@ def test_probe_field_tracing_matches_oracle():
 
 @pytest.mark.parametrize("persistent", [0, 1, 2])
 @pytest.mark.parametrize("blocks", [1, 8])
-def test_dispatch_modes_identical(persistent, blocks):
-    _print(f"case: dispatch persistent={persistent} blocks_per_cu={blocks}, spheres cut + soup bounce rays")
+@pytest.mark.parametrize("sort", [0, 1])
+def test_dispatch_modes_identical(persistent, blocks, sort):
+    _print(f"case: dispatch persistent={persistent} blocks_per_cu={blocks} sort={sort}, spheres cut + soup bounce rays")
     d = G.dev()
-    keep = (d.get_option(B.RTQ_OPTION_PERSISTENT), d.get_option(B.RTQ_OPTION_BLOCKS_PER_CU))
+    keep = (d.get_option(B.RTQ_OPTION_PERSISTENT), d.get_option(B.RTQ_OPTION_BLOCKS_PER_CU),
+            d.get_option(B.RTQ_OPTION_SORT_RAYS))
     d.set_option(B.RTQ_OPTION_PERSISTENT, persistent)
     d.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, blocks)
+    d.set_option(B.RTQ_OPTION_SORT_RAYS, sort)
     try:
         s = G.scene(2, 1 / 16)
         f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 400 / 300)
@@ -144,6 +147,7 @@ def test_dispatch_modes_identical(persistent, blocks):
     finally:
         d.set_option(B.RTQ_OPTION_PERSISTENT, keep[0])
         d.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, keep[1])
+        d.set_option(B.RTQ_OPTION_SORT_RAYS, keep[2])
 
 
 def test_box_scene_any_hit():

@@ -29,6 +29,7 @@ RTQ_SHADOW_SUN = 1
 RTQ_CAMERA_REFERENCE_ROWS = 1
 RTQ_OPTION_PERSISTENT = 1
 RTQ_OPTION_BLOCKS_PER_CU = 2
+RTQ_OPTION_SORT_RAYS = 3
 RTQ_NO_HIT = 0xFFFFFFFF
 RTQ_PROBE_RAYS = 64
 RTQ_BATCH_REFILL = 1

@@ -47,17 +47,20 @@ __device__ __forceinline__ float boxHit(const Ray3& r, float mnx, float mny, flo
   return (tmin < tm) ? tmin : -1.0f;
 }
 
-// ---- slab division without the per-quotient reciprocal -----------------------------------
-// IEEE fp32 x / d compiles to v_div_scale (x2), v_rcp, fma, fma, mul, fma, fma, fma, v_div_fmas,
-// v_div_fixup. When no operand needs rescaling (v_div_scale returns its input, VCC = 0) and x is
-// finite and non-zero, v_div_fmas is a plain fma and v_div_fixup returns the fma result, so the
-// quotient is fully determined by the refined reciprocal r(d) = fma(fma(-d, rcp(d), 1), rcp(d),
-// rcp(d)) and five fma-class operations. r(d) depends only on the ray, so it is computed once per
-// ray. Conditions that keep v_div_scale an identity: |d| in [2^-30, 2^30] and every non-zero
-// x = bound - origin in [2^-60, 2^60]; the latter holds when every box bound and origin coordinate
-// is 0 or has magnitude in [2^-37, 2^59] (differences of such values are 0 or >= 2^-60). For
-// x = +-0 the quotient may differ from IEEE in the sign of zero only; CollideBox returns
-// max(0, ...) and compares with <, so that sign never changes its result.
+// ---- slab division without the per-quotient division sequence -----------------------------
+// CollideBox divides, so bit parity needs the IEEE fp32 quotient RN(x / d) of every slab, x = bound -
+// origin (rounded in fp32 as the shader does). An fp32 division compiles to about ten instructions
+// with a quarter-rate reciprocal; here it is one fp64 multiply by the ray's fp64 reciprocal and two
+// conversions: q = RN32(RN64(double(x) * RN64(1 / double(d)))).
+// Why that is exact: RN64(1/d) and the rounded product each carry a relative error below 2^-53, so
+// the fp64 value is within 2^-52 (relative) of x / d. A quotient of two 24-bit significands is never a
+// midpoint between adjacent fp32 values, and its distance to one is at least 1 / (D (2K+1)) >= 2^-49
+// relative (D the divisor's significand, 2K+1 the midpoint's, both integers below 2^24 and 2^25), so the
+// fp64 value and x / d round to the same fp32 number. The fp64 reciprocal is computed once per ray.
+// Domain, which keeps every quotient a normal fp32 number: |d| in [2^-30, 2^30] and every non-zero
+// x in [2^-60, 2^60]; the latter holds when every box bound and origin coordinate is 0 or has
+// magnitude in [2^-37, 2^59] (differences of such values are 0 or >= 2^-60). x = +-0 gives the IEEE
+// signed zero. Lanes outside the domain take the IEEE division.
 __device__ __forceinline__ bool fastDirComponent(float d) {
   const float a = fabsf(d);
   return a >= 0x1p-30f && a <= 0x1p30f;
@@ -66,21 +69,11 @@ __device__ __forceinline__ bool fastCoord(float v) {
   const float a = fabsf(v);
   return a == 0.0f || (a >= 0x1p-37f && a <= 0x1p59f);
 }
-__device__ __forceinline__ float refinedRcp(float d) {
-  const float r0 = __builtin_amdgcn_rcpf(d);
-  const float e = __builtin_fmaf(-d, r0, 1.0f);
-  return __builtin_fmaf(e, r0, r0);
-}
-__device__ __forceinline__ float quotient(float x, float d, float r) {
-  float q = x * r;
-  float t = __builtin_fmaf(-d, q, x);
-  q = __builtin_fmaf(t, r, q);
-  t = __builtin_fmaf(-d, q, x);
-  return __builtin_fmaf(t, r, q);
-}
+__device__ __forceinline__ double recip64(float d) { return 1.0 / double(d); }
+__device__ __forceinline__ float quotient(float x, double r) { return float(double(x) * r); }
 
 struct RayRcp {
-  float rx, ry, rz;
+  double rx, ry, rz;
 };
 
 // CollideBox with quotient(); no NaN can arise (finite x, non-zero d, no overflow), so min/max of
@@ -88,10 +81,8 @@ struct RayRcp {
 // to the sign of a zero result, which the final max(0, .) and the < comparison ignore.
 __device__ __forceinline__ float boxHitFast(const Ray3& r, const RayRcp& q, float mnx, float mny, float mnz,
                                             float mxx, float mxy, float mxz, float tmax) {
-  const float t0x = quotient(mnx - r.ox, r.dx, q.rx), t0y = quotient(mny - r.oy, r.dy, q.ry),
-              t0z = quotient(mnz - r.oz, r.dz, q.rz);
-  const float t1x = quotient(mxx - r.ox, r.dx, q.rx), t1y = quotient(mxy - r.oy, r.dy, q.ry),
-              t1z = quotient(mxz - r.oz, r.dz, q.rz);
+  const float t0x = quotient(mnx - r.ox, q.rx), t0y = quotient(mny - r.oy, q.ry), t0z = quotient(mnz - r.oz, q.rz);
+  const float t1x = quotient(mxx - r.ox, q.rx), t1y = quotient(mxy - r.oy, q.ry), t1z = quotient(mxz - r.oz, q.rz);
   const float tmin = fmaxf(0.0f, fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z))));
   const float tm = fminf(tmax, fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z))));
   return (tmin < tm) ? tmin : -1.0f;
@@ -142,7 +133,7 @@ template <bool ANY, bool COUNT, int STACK, bool FAST>
 __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const Ray3& r, float tmax, Closest& hit,
                                          Counters& cnt) {
   RayRcp rq{};
-  if (FAST) rq = RayRcp{refinedRcp(r.dx), refinedRcp(r.dy), refinedRcp(r.dz)};
+  if (FAST) rq = RayRcp{recip64(r.dx), recip64(r.dy), recip64(r.dz)};
   uint32_t node = s.root;
   int sp = 0;
   if (!ANY) hit.t = tmax;
@@ -220,13 +211,17 @@ __device__ __forceinline__ Ray3 loadRay(const float4* rays, uint32_t i, float& t
   return Ray3{a.x, a.y, a.z, b.x, b.y, b.z};
 }
 
+// perm (optional): lane order -> ray index, so that neighbouring lanes trace rays a sort has put
+// together; results still land at the ray's own index.
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s, const float4* __restrict__ rays, uint32_t n,
                                                              float4* __restrict__ hits, uint32_t* __restrict__ leafNid,
-                                                             uint32_t* __restrict__ counters) {
+                                                             uint32_t* __restrict__ counters,
+                                                             const uint32_t* __restrict__ perm) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
-  const uint32_t i = xcdBlock() * kTraceBlock + threadIdx.x;
-  if (i >= n) return;
+  const uint32_t lanePos = xcdBlock() * kTraceBlock + threadIdx.x;
+  if (lanePos >= n) return;
+  const uint32_t i = perm ? perm[lanePos] : lanePos;
   float tmax;
   const Ray3 r = loadRay(rays, i, tmax);
   Closest h{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
@@ -243,10 +238,12 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s, const f
 
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s, const float4* __restrict__ rays, uint32_t n,
-                                                         uint32_t* __restrict__ occluded, uint32_t* __restrict__ counters) {
+                                                         uint32_t* __restrict__ occluded, uint32_t* __restrict__ counters,
+                                                         const uint32_t* __restrict__ perm) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
-  const uint32_t i = xcdBlock() * kTraceBlock + threadIdx.x;
-  if (i >= n) return;
+  const uint32_t lanePos = xcdBlock() * kTraceBlock + threadIdx.x;
+  if (lanePos >= n) return;
+  const uint32_t i = perm ? perm[lanePos] : lanePos;
   float tmax;
   const Ray3 r = loadRay(rays, i, tmax);
   Closest h{};
@@ -306,7 +303,7 @@ __device__ __forceinline__ void descend(const DevNode* __restrict__ nodes, uint3
 template <int STACK, bool ANY, bool RANGE>
 __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, const float4* __restrict__ rays,
                                             uint32_t n, void* __restrict__ out, uint32_t* __restrict__ leafNid,
-                                            uint32_t* __restrict__ counter) {
+                                            uint32_t* __restrict__ counter, const uint32_t* __restrict__ perm) {
   const uint32_t lane = threadIdx.x & 63u;
   bool active = false, fast = false, exhausted = false;
   uint32_t rid = 0, node = kRefDone;
@@ -345,10 +342,10 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
         if (!active) {
           const uint32_t k = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
           if (k < next) {
-            rid = k;
-            r = loadRay(rays, k, tmax);
+            rid = perm ? perm[k] : k;
+            r = loadRay(rays, rid, tmax);
             fast = fastLane(s, r, tmax);
-            if (fast) rq = RayRcp{refinedRcp(r.dx), refinedRcp(r.dy), refinedRcp(r.dz)};
+            if (fast) rq = RayRcp{recip64(r.dx), recip64(r.dy), recip64(r.dz)};
             hit = Closest{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
             node = s.root;
             sp = 0;
@@ -404,17 +401,46 @@ template <int STACK, bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void kTracePersistent(DevScene s, const float4* __restrict__ rays,
                                                                 uint32_t n, void* __restrict__ out,
                                                                 uint32_t* __restrict__ leafNid,
-                                                                uint32_t* __restrict__ counter) {
+                                                                uint32_t* __restrict__ counter,
+                                                                const uint32_t* __restrict__ perm) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
-  refillTrace<STACK, ANY, false>(s, lds + threadIdx.x, rays, n, out, leafNid, counter);
+  refillTrace<STACK, ANY, false>(s, lds + threadIdx.x, rays, n, out, leafNid, counter, perm);
 }
 
 template <int STACK, bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void kTraceWaveRange(DevScene s, const float4* __restrict__ rays,
                                                                uint32_t n, void* __restrict__ out,
-                                                               uint32_t* __restrict__ leafNid) {
+                                                               uint32_t* __restrict__ leafNid,
+                                                               const uint32_t* __restrict__ perm) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
-  refillTrace<STACK, ANY, true>(s, lds + threadIdx.x, rays, n, out, leafNid, nullptr);
+  refillTrace<STACK, ANY, true>(s, lds + threadIdx.x, rays, n, out, leafNid, nullptr, perm);
+}
+
+// ---- ray sort keys -------------------------------------------------------------------------------
+// Key of ray i: direction octant in bits 27-29 above a 27-bit Morton code of the origin quantised to
+// 512 cells per axis over the scene box (out-of-box and NaN coordinates clamp to the border cells).
+// Sorting by it puts rays that start close together and head the same way on neighbouring lanes.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 9 bits -> every third of 27 bits
+  v &= 0x1FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__global__ __launch_bounds__(256) void kRayKeys(const float4* __restrict__ rays, uint32_t n, float mnx, float mny,
+                                                float mnz, float sx, float sy, float sz, uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const float4 o = rays[2 * size_t(i)], d = rays[2 * size_t(i) + 1];
+  const float q[3] = {(o.x - mnx) * sx, (o.y - mny) * sy, (o.z - mnz) * sz};
+  uint32_t c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c[k] = q[k] >= 0.0f ? (q[k] < 511.0f ? uint32_t(q[k]) : 511u) : 0u;
+  const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+  keys[i] = (oct << 27) | (spread3(c[0]) << 2) | (spread3(c[1]) << 1) | spread3(c[2]);
+  vals[i] = i;
 }
 
 // ---- camera rays generated in the kernel --------------------------------------------------
@@ -791,7 +817,7 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
           tris = s.tris;
           r = loadRay(it.rays, rid, tmax);
           fast = fastLane(s, r, tmax);
-          if (fast) rq = RayRcp{refinedRcp(r.dx), refinedRcp(r.dy), refinedRcp(r.dz)};
+          if (fast) rq = RayRcp{recip64(r.dx), recip64(r.dy), recip64(r.dz)};
           anyRay = it.flags != 1u;
           hit = Closest{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};  // any hit keeps t = tmax throughout
           node = s.root;
@@ -876,7 +902,7 @@ __global__ __launch_bounds__(256) void kDivSelfTest(uint64_t seed, uint32_t perT
     const float d = __uint_as_float(((a >> 31) << 31) | (ed << 23) | md);
     const float x = __uint_as_float(((b >> 31) << 31) | (ex << 23) | mx);
     const float ieee = x / d;
-    const float fast = quotient(x, d, refinedRcp(d));
+    const float fast = quotient(x, recip64(d));
     bad += (__float_as_uint(ieee) != __float_as_uint(fast)) ? 1ull : 0ull;
   }
   atomicAdd(mismatches, bad);

@@ -12,6 +12,10 @@
 #include <unordered_map>
 #include <vector>
 
+#include <mutex>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "../../../include/rtq.h"
 #include "scene_layout.h"
 #include "kernels.h"
@@ -22,6 +26,12 @@ struct rtq_device {
   int numCU = 0;
   uint32_t persistent = 0;
   uint32_t blocksPerCU = 8;
+  uint32_t sortRays = 0;
+  // ray-sort workspace (keys, values and rocPRIM's temporary storage), grown on demand and shared by
+  // every sorting dispatch on this device
+  std::mutex sortMu;
+  void* sortBuf = nullptr;
+  size_t sortBytes = 0;
 };
 
 // Work counters for persistent dispatches: a small ring so that dispatches in flight on different
@@ -30,6 +40,7 @@ constexpr uint32_t kCounterRing = 64;
 
 struct rtq_scene {
   rtq_device* dev = nullptr;
+  float boxMin[3] = {0, 0, 0}, boxMax[3] = {0, 0, 0};  // bounds of the vertex positions (ray sort keys)
   rtq::DevScene view{};
   rtq_scene_info info{};
   std::vector<void*> allocations;
@@ -132,6 +143,11 @@ rtq_status rtq_device_create(int hip_ordinal, rtq_device** out) {
 }
 
 rtq_status rtq_device_destroy(rtq_device* dev) {
+  if (dev && dev->sortBuf) {
+    hipSetDevice(dev->ordinal);
+    hipDeviceSynchronize();
+    hipFree(dev->sortBuf);
+  }
   delete dev;
   return RTQ_OK;
 }
@@ -147,6 +163,10 @@ rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t valu
       if (value < 1 || value > 16) return fail(RTQ_ERR_INVALID_ARGUMENT, "blocks per CU in 1..16");
       dev->blocksPerCU = value;
       return RTQ_OK;
+    case RTQ_OPTION_SORT_RAYS:
+      if (value > 1) return fail(RTQ_ERR_INVALID_ARGUMENT, "sort is 0 or 1");
+      dev->sortRays = value;
+      return RTQ_OK;
   }
   return fail(RTQ_ERR_INVALID_ARGUMENT, "unknown option");
 }
@@ -156,6 +176,7 @@ rtq_status rtq_device_get_option(const rtq_device* dev, uint32_t option, uint32_
   switch (option) {
     case RTQ_OPTION_PERSISTENT: *value = dev->persistent; return RTQ_OK;
     case RTQ_OPTION_BLOCKS_PER_CU: *value = dev->blocksPerCU; return RTQ_OK;
+    case RTQ_OPTION_SORT_RAYS: *value = dev->sortRays; return RTQ_OK;
   }
   return fail(RTQ_ERR_INVALID_ARGUMENT, "unknown option");
 }
@@ -303,6 +324,16 @@ rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, 
     if ((st = uploadArray(s, std::vector<uint32_t>(kCounterRing, 0u), &dc)) != RTQ_OK) return bail(st);
     s->counters = const_cast<uint32_t*>(dc);
   }
+  for (int k = 0; k < 3; ++k) {
+    s->boxMin[k] = numVertices ? positions[k] : 0.0f;
+    s->boxMax[k] = s->boxMin[k];
+  }
+  for (uint32_t v = 0; v < numVertices; ++v)
+    for (int k = 0; k < 3; ++k) {
+      const float x = positions[3 * size_t(v) + k];
+      if (x < s->boxMin[k]) s->boxMin[k] = x;
+      if (x > s->boxMax[k]) s->boxMax[k] = x;
+    }
   s->view.root = h.root;
   s->view.numNodes = h.numNodes;
   s->view.leafWords = h.leafWords;
@@ -329,6 +360,48 @@ rtq_status rtq_scene_get_info(const rtq_scene* scene, rtq_scene_info* info) {
   return RTQ_OK;
 }
 
+// Sort keys and rocPRIM radix sort on `st`; *perm receives the device permutation (lane order -> ray).
+// The lock keeps the device's workspace for this dispatch until its kernels are enqueued; enqueued work
+// of two streams could still overlap, so sorting dispatches on one device must share a stream.
+static rtq_status sortRays(rtq_scene* s, const float4* rays, uint32_t n, hipStream_t st,
+                           std::unique_lock<std::mutex>& lock, const uint32_t** perm) {
+  rtq_device* dev = s->dev;
+  lock = std::unique_lock<std::mutex>(dev->sortMu);
+  size_t tempBytes = 0;
+  RTQ_HIP(rocprim::radix_sort_pairs(nullptr, tempBytes, static_cast<const uint32_t*>(nullptr),
+                                    static_cast<uint32_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
+                                    static_cast<uint32_t*>(nullptr), n, 0u, 30u, st));
+  const size_t arr = (size_t(n) * 4 + 255) & ~size_t(255);
+  const size_t need = 4 * arr + tempBytes;
+  if (need > dev->sortBytes) {
+    if (dev->sortBuf) {
+      RTQ_HIP(hipStreamSynchronize(st));
+      RTQ_HIP(hipFree(dev->sortBuf));
+      dev->sortBuf = nullptr;
+      dev->sortBytes = 0;
+    }
+    RTQ_HIP(hipMalloc(&dev->sortBuf, need));
+    dev->sortBytes = need;
+  }
+  uint8_t* base = static_cast<uint8_t*>(dev->sortBuf);
+  uint32_t* keysIn = reinterpret_cast<uint32_t*>(base);
+  uint32_t* keysOut = reinterpret_cast<uint32_t*>(base + arr);
+  uint32_t* valsIn = reinterpret_cast<uint32_t*>(base + 2 * arr);
+  uint32_t* valsOut = reinterpret_cast<uint32_t*>(base + 3 * arr);
+  void* temp = base + 4 * arr;
+  float sc[3];
+  for (int k = 0; k < 3; ++k) {
+    const float ext = s->boxMax[k] - s->boxMin[k];
+    sc[k] = ext > 0.0f ? 512.0f / ext : 0.0f;
+  }
+  hipLaunchKernelGGL(rtq::kRayKeys, dim3((n + 255) / 256), dim3(256), 0, st, rays, n, s->boxMin[0], s->boxMin[1],
+                     s->boxMin[2], sc[0], sc[1], sc[2], keysIn, valsIn);
+  RTQ_HIP(hipGetLastError());
+  RTQ_HIP(rocprim::radix_sort_pairs(temp, tempBytes, keysIn, keysOut, valsIn, valsOut, n, 0u, 30u, st));
+  *perm = valsOut;
+  return RTQ_OK;
+}
+
 static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32_t flags, void* out, uint32_t* nid,
                            uint32_t* counters, void* stream) {
   if (!s) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene");
@@ -339,15 +412,21 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(rays);
+  const uint32_t* perm = nullptr;
+  std::unique_lock<std::mutex> sortLock;
+  if (!counters && s->dev->sortRays) {
+    rtq_status ss = sortRays(s, r, n, st, sortLock, &perm);
+    if (ss != RTQ_OK) return ss;
+  }
   if (!counters && s->dev->persistent == 2) {
     const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
     const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, n + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
     withStack(s->info.max_depth, [&](auto S) {
       constexpr int K = decltype(S)::value;
       if (flags == RTQ_TRACE_CLOSEST)
-        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid);
+        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid, perm);
       else
-        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, true>), pgrid, block, 0, st, s->view, r, n, out, nullptr);
+        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, true>), pgrid, block, 0, st, s->view, r, n, out, nullptr, perm);
     });
     RTQ_HIP(hipGetLastError());
     return RTQ_OK;
@@ -360,10 +439,11 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
     withStack(s->info.max_depth, [&](auto S) {
       constexpr int K = decltype(S)::value;
       if (flags == RTQ_TRACE_CLOSEST)
-        hipLaunchKernelGGL((rtq::kTracePersistent<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid, counter);
+        hipLaunchKernelGGL((rtq::kTracePersistent<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid, counter,
+                           perm);
       else
         hipLaunchKernelGGL((rtq::kTracePersistent<K, true>), pgrid, block, 0, st, s->view, r, n, out, nullptr,
-                           counter);
+                           counter, perm);
     });
     RTQ_HIP(hipGetLastError());
     return RTQ_OK;
@@ -372,12 +452,16 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
     constexpr int K = decltype(S)::value;
     if (flags == RTQ_TRACE_CLOSEST) {
       float4* h = static_cast<float4*>(out);
-      if (counters) hipLaunchKernelGGL((rtq::kTraceClosest<K, true>), grid, block, 0, st, s->view, r, n, h, nid, counters);
-      else hipLaunchKernelGGL((rtq::kTraceClosest<K, false>), grid, block, 0, st, s->view, r, n, h, nid, nullptr);
+      if (counters)
+        hipLaunchKernelGGL((rtq::kTraceClosest<K, true>), grid, block, 0, st, s->view, r, n, h, nid, counters, nullptr);
+      else
+        hipLaunchKernelGGL((rtq::kTraceClosest<K, false>), grid, block, 0, st, s->view, r, n, h, nid, nullptr, perm);
     } else {
       uint32_t* o = static_cast<uint32_t*>(out);
-      if (counters) hipLaunchKernelGGL((rtq::kTraceAny<K, true>), grid, block, 0, st, s->view, r, n, o, counters);
-      else hipLaunchKernelGGL((rtq::kTraceAny<K, false>), grid, block, 0, st, s->view, r, n, o, nullptr);
+      if (counters)
+        hipLaunchKernelGGL((rtq::kTraceAny<K, true>), grid, block, 0, st, s->view, r, n, o, counters, nullptr);
+      else
+        hipLaunchKernelGGL((rtq::kTraceAny<K, false>), grid, block, 0, st, s->view, r, n, o, nullptr, perm);
     }
   });
   RTQ_HIP(hipGetLastError());

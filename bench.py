@@ -63,6 +63,8 @@ def parse():
                     help="persistent: lanes refill from all rays; range: each wave refills from its own "
                          "contiguous range; grid: one lane per ray; auto: grid for the coherent primary "
                          "rays, persistent for incoherent bounces")
+    ap.add_argument("--leaf-wait", type=int, default=0,
+                    help="lanes ready for leaf tests that end a round of descent, 1..64 (0 = library default)")
     ap.add_argument("--sort", type=int, choices=(0, 1), default=0,
                     help="primary/incoherent: sort rays by direction octant and origin cell before tracing")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent dispatch: blocks per CU (0 = library "
@@ -91,9 +93,11 @@ def configure_dispatch(dev, args, auto: str):
         args.dispatch = auto
     dev.set_option(B.RTQ_OPTION_PERSISTENT, {"grid": 0, "persistent": 1, "range": 2}[args.dispatch])
     dev.set_option(B.RTQ_OPTION_SORT_RAYS, args.sort)
+    dev.set_option(B.RTQ_OPTION_LEAF_WAIT, args.leaf_wait)
     if args.blocks_per_cu:
         dev.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, args.blocks_per_cu)
-    return {"mode": args.dispatch, "blocks_per_cu": dev.get_option(B.RTQ_OPTION_BLOCKS_PER_CU), "sort_rays": args.sort}
+    return {"mode": args.dispatch, "blocks_per_cu": dev.get_option(B.RTQ_OPTION_BLOCKS_PER_CU), "sort_rays": args.sort,
+            "leaf_wait": args.leaf_wait}
 
 
 def kernel_name(args, depth: int, any_hit: bool = False) -> str:
@@ -387,6 +391,8 @@ def _scene_pool(sizes, distinct, threads, cache_dir=None):
             b.save_cache(path)
         blob, ranges = b.fill_gpu_buffer()
         pool.append((b, blob, ranges, b.positions()))
+        if (i + 1) % 16 == 0:
+            print(f"built {i + 1} of {min(distinct, len(sizes))} scenes", file=sys.stderr, flush=True)
     return pool
 
 
@@ -444,6 +450,7 @@ def bench_batch(args):
         return ev0.elapsed_ms(ev1) / args.steps
 
     plain_ms = timed(lambda: batch.launch_plain(sptr))
+    print(f"plain: {plain_ms:.3f} ms per batch; autotuning", file=sys.stderr, flush=True)
     best, tuned_ms = batch.autotune(reps=2, stream=sptr)
     batch.set_params(use_graph=1)
     batched_ms = timed(lambda: batch.launch(sptr))
@@ -460,7 +467,7 @@ def bench_batch(args):
         "plain_mrays_s": round(total / (plain_ms * 1e-3) / 1e6, 2), "plain_ms": round(plain_ms, 4),
         "speedup_vs_plain": round(plain_ms / batched_ms, 3),
         "autotuned": {"chunk_rays": p.chunk_rays, "blocks_per_cu": p.blocks_per_cu, "stack_depth": p.stack_depth,
-                      "use_graph": p.use_graph, "tuning_ms": round(tuned_ms, 4)},
+                      "use_graph": p.use_graph, "leaf_wait": p.leaf_wait, "tuning_ms": round(tuned_ms, 4)},
         "host": {"threads": threads, "build_s": round(build_s, 1)},
     }), flush=True)
 
@@ -504,7 +511,7 @@ def bench_serve(args):
     out_off = np.concatenate([[0], np.cumsum(np.where(req_any, 4, 16) * req_sizes)]).astype(np.int64)
     out_arena = dev.host_alloc(int(out_off[-1]))
     budget = 2 << 30
-    params = B.BatchParams(args.chunk_rays, args.blocks_per_cu or 8, stack_class(max_depth), 0)
+    params = B.BatchParams(args.chunk_rays, args.blocks_per_cu or 8, stack_class(max_depth), 0, args.leaf_wait)
 
     def run(plain, arrivals):
         srv = D.Server(dev, scenes, budget_bytes=budget, plain=plain, max_batch_rays=args.max_batch_rays,

@@ -121,6 +121,11 @@ rtq_status rtq_device_destroy(rtq_device* dev);
  * each ray's own index and are identical. The sort workspace belongs to the device and grows on
  * demand: sorting dispatches on one device must be issued on one stream. Default 0. */
 #define RTQ_OPTION_SORT_RAYS 3u
+/* RTQ_OPTION_LEAF_WAIT: a wave interleaves descent toward the next leaf with the tests of the leaves
+ * its lanes have reached; a round of descent ends once this many of its 64 lanes are ready for leaf
+ * tests (1..64; 64 = every lane reaches its leaf first). 0 (default) = 24 for one ray per lane, 32 for
+ * refilling dispatches. Results are identical for every value. */
+#define RTQ_OPTION_LEAF_WAIT 4u
 rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t value);
 rtq_status rtq_device_get_option(const rtq_device* dev, uint32_t option, uint32_t* value);
 
@@ -258,6 +263,8 @@ typedef struct rtq_batch_params {
   uint32_t blocks_per_cu; /* persistent 128-lane blocks per compute unit, 1..16 */
   uint32_t stack_depth;   /* per-lane LDS stack entries: 24, 32 or 64, >= the deepest scene */
   uint32_t use_graph;     /* 1: replay a captured hipGraph (needs a non-NULL stream) */
+  uint32_t leaf_wait;     /* lanes ready for leaf tests that end a round of descent (RTQ_OPTION_LEAF_WAIT);
+                             0 = default */
 } rtq_batch_params;
 
 rtq_status rtq_batch_create(rtq_device* dev, rtq_scene* const* scenes, uint32_t num_scenes,

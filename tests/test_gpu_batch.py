@@ -84,11 +84,12 @@ def test_batch_matches_plain_and_oracle():
             assert np.array_equal(h, occ)
     stream = dev.stream()
     sptr = stream.ptr
-    for chunk, blocks, stack, graph in ((64, 2, 24, 0), (128, 4, 32, 0), (512, 8, 64, 0), (256, 6, 32, 1),
-                                        (1, 8, 24, 0), (1, 2, 64, 0), (1, 12, 32, 1)):
+    for chunk, blocks, stack, graph, wait in ((64, 2, 24, 0, 0), (128, 4, 32, 0, 5), (512, 8, 64, 0, 64),
+                                              (256, 6, 32, 1, 0), (1, 8, 24, 0, 0), (1, 2, 64, 0, 1),
+                                              (1, 12, 32, 1, 48), (1, 8, 24, 0, 64)):
         for (d_r, d_o, d_n) in bufs:
             d_o.write(np.zeros(d_o.nbytes, np.uint8))
-        batch.set_params(chunk_rays=chunk, blocks_per_cu=blocks, stack_depth=stack, use_graph=graph)
+        batch.set_params(chunk_rays=chunk, blocks_per_cu=blocks, stack_depth=stack, use_graph=graph, leaf_wait=wait)
         for _ in range(3 if graph else 1):   # graph replays must all give the same result
             batch.launch(sptr)
             stream.synchronize()
@@ -98,7 +99,7 @@ def test_batch_matches_plain_and_oracle():
                 if a[1] is not None:
                     assert np.array_equal(a[1], b[1])
     best, ms = batch.autotune(reps=1, stream=sptr)
-    assert best.chunk_rays in (1, 64, 128, 256, 512) and ms > 0
+    assert best.chunk_rays in (1, 64, 128, 256, 512) and best.leaf_wait <= 64 and ms > 0
     batch.launch(sptr)
     stream.synchronize()
     got = read_all()

@@ -150,6 +150,36 @@ def test_dispatch_modes_identical(persistent, blocks, sort):
         d.set_option(B.RTQ_OPTION_SORT_RAYS, keep[2])
 
 
+@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("wait", [1, 7, 40, 64])
+def test_leaf_wait_identical(persistent, wait):
+    """Rounds of descent end at any number of lanes waiting at a leaf: results do not change."""
+    _print(f"case: leaf wait {wait}, persistent={persistent}, soup bounce rays and spheres cut")
+    d = G.dev()
+    keep = (d.get_option(B.RTQ_OPTION_PERSISTENT), d.get_option(B.RTQ_OPTION_LEAF_WAIT))
+    d.set_option(B.RTQ_OPTION_PERSISTENT, persistent)
+    d.set_option(B.RTQ_OPTION_LEAF_WAIT, wait)
+    try:
+        s3 = G.scene(3, 1 / 48)
+        rays3 = s3["builder"].surface_rays(seed=9, count=50_000)
+        G.assert_closest_equal(s3["gpu"].trace_closest(rays3), s3["orc"].trace_closest(rays3, threads=16), "soup")
+        assert np.array_equal(s3["gpu"].trace_any(rays3), s3["orc"].trace_any(rays3, threads=16)[0])
+        s = G.scene(2, 1 / 16)
+        f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 400 / 300)
+        rays = H.frustum_rays(f, 400, 300, seed=11)
+        G.assert_closest_equal(s["gpu"].trace_closest(rays), s["orc"].trace_closest(rays, threads=16), "spheres")
+    finally:
+        d.set_option(B.RTQ_OPTION_PERSISTENT, keep[0])
+        d.set_option(B.RTQ_OPTION_LEAF_WAIT, keep[1])
+
+
+def test_leaf_wait_option_bounds():
+    d = G.dev()
+    with pytest.raises(Exception):
+        d.set_option(B.RTQ_OPTION_LEAF_WAIT, 65)
+    assert d.get_option(B.RTQ_OPTION_LEAF_WAIT) == 0
+
+
 def test_box_scene_any_hit():
     _print("case: closed box any-hit, random t limits")
     s = G.scene(1)

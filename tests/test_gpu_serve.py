@@ -53,7 +53,7 @@ def test_serve_mix_equals_alone():
             alone.append((scenes[si].trace_any(rays), None))
     # mixed: a budget that holds only part of the stream at once, 4 submitting threads; chunked and
     # lane-refilling batch kernels
-    for params in (None, B.BatchParams(B.RTQ_BATCH_REFILL, 8, 64, 0)):
+    for params in (None, B.BatchParams(B.RTQ_BATCH_REFILL, 8, 64, 0, 64)):
         srv = D.Server(dev, scenes, budget_bytes=8 << 20, params=params)
         handles = [None] * len(reqs)
 

@@ -30,6 +30,7 @@ RTQ_CAMERA_REFERENCE_ROWS = 1
 RTQ_OPTION_PERSISTENT = 1
 RTQ_OPTION_BLOCKS_PER_CU = 2
 RTQ_OPTION_SORT_RAYS = 3
+RTQ_OPTION_LEAF_WAIT = 4
 RTQ_NO_HIT = 0xFFFFFFFF
 RTQ_PROBE_RAYS = 64
 RTQ_BATCH_REFILL = 1
@@ -72,7 +73,7 @@ class BatchItem(C.Structure):
 
 class BatchParams(C.Structure):
     _fields_ = [("chunk_rays", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("stack_depth", C.c_uint32),
-                ("use_graph", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("leaf_wait", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

@@ -88,7 +88,14 @@ __device__ __forceinline__ float boxHitFast(const Ray3& r, const RayRcp& q, floa
   return (tmin < tm) ? tmin : -1.0f;
 }
 
-// CollideTriangle (collision.glsl:58-95) with edges precomputed; u, v written when accepted.
+// CollideTriangle (collision.glsl:58-95) with edges precomputed. Every quantity is computed with the
+// shader's operations in its order, and the early returns are folded into one final condition, so the
+// wave runs one straight-line block (its loads issue together) instead of three divergent branches; an
+// accepted result is the same bits either way. The conditions, as the shader tests them:
+//   |a| < EPSILON rejects (NaN a is not rejected here, but then u is NaN and rejected below);
+//   clamp(u, 0, 1) != u rejects, with GLSL min/max this is exactly !(u >= 0 && u <= 1) (NaN fails both,
+//   -0 passes both); v < 0 or u + v > 1 rejects; t > EPSILON && t < tmax accepts.
+// u and v are written in every case; callers read them only when the result is > 0.
 __device__ __forceinline__ float triHit(const Ray3& r, const float4 t0, const float4 t1, const float4 t2, float tmax,
                                         float& uo, float& vo) {
   const float p0x = t0.x, p0y = t0.y, p0z = t0.z;
@@ -98,23 +105,19 @@ __device__ __forceinline__ float triHit(const Ray3& r, const float4 t0, const fl
   const float hy = r.dz * e2x - r.dx * e2z;
   const float hz = r.dx * e2y - r.dy * e2x;
   const float a = (e1x * hx + e1y * hy) + e1z * hz;
-  if (fabsf(a) < kEps) return -1.0f;
   const float f = 1.0f / a;
   const float sx = r.ox - p0x, sy = r.oy - p0y, sz = r.oz - p0z;
   const float u = f * ((sx * hx + sy * hy) + sz * hz);
-  if (gmin(gmax(u, 0.0f), 1.0f) != u) return -1.0f;
   const float qx = sy * e1z - sz * e1y;
   const float qy = sz * e1x - sx * e1z;
   const float qz = sx * e1y - sy * e1x;
   const float v = f * ((r.dx * qx + r.dy * qy) + r.dz * qz);
-  if (v < 0.0f || u + v > 1.0f) return -1.0f;
   const float t = f * ((e2x * qx + e2y * qy) + e2z * qz);
-  if (t > kEps && t < tmax) {
-    uo = u;
-    vo = v;
-    return t;
-  }
-  return -1.0f;
+  const bool accept = !(fabsf(a) < kEps) && u >= 0.0f && u <= 1.0f && !(v < 0.0f) && !(u + v > 1.0f) && t > kEps &&
+                      t < tmax;
+  uo = u;
+  vo = v;
+  return accept ? t : -1.0f;
 }
 
 struct Counters {
@@ -127,25 +130,38 @@ struct Closest {
   float u, v;
 };
 
-// Walk the tree for one ray. ANY: stop at the first accepted triangle (returns true).
-// `stk` points at this lane's column of the block's LDS stack (stride kTraceBlock).
-template <bool ANY, bool COUNT, int STACK, bool FAST>
-__device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const Ray3& r, float tmax, Closest& hit,
-                                         Counters& cnt) {
-  RayRcp rq{};
-  if (FAST) rq = RayRcp{recip64(r.dx), recip64(r.dy), recip64(r.dz)};
-  uint32_t node = s.root;
-  int sp = 0;
-  if (!ANY) hit.t = tmax;
+// ---- descent rounds ----------------------------------------------------------------------------
+// A wave alternates between descent (each lane steps down inner nodes toward its next leaf) and a
+// leaf phase (lanes at a leaf test its triangles). If every descent ran until all lanes had reached a
+// leaf, the lanes that got there first would idle while the deepest one descends, and on incoherent
+// rays most instructions would run with a few lanes on. So a round of descent ends as soon as `wait`
+// lanes of the wave are ready for the leaf phase (at a leaf or finished, plus `extra` lanes that are
+// ready anyway, such as free lanes about to take a new ray); lanes still at an inner node keep their
+// node and stack and go on in the next round. Each ray's own sequence of operations, and so every
+// result, is the same for any `wait`; only the interleaving of the lanes changes. wait = 64 gives the
+// plain order (every lane reaches its leaf before any leaf is tested).
+constexpr uint32_t kGridWait = 24;    // default when each lane traces one ray
+constexpr uint32_t kRefillWait = 32;  // default when lanes refill with new rays
+
+// One round of descent for the calling lanes. FASTONLY: every calling lane takes the reciprocal slab
+// form (decided per wave), so no per-lane branch is needed; otherwise `fast` chooses per lane.
+template <int STACK, bool FASTONLY, bool COUNT>
+__device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, uint32_t* stk, const Ray3& r,
+                                             const RayRcp& rq, bool fast, float lim, uint32_t& node, int& sp,
+                                             uint32_t wait, uint32_t extra, uint32_t& inner) {
+  const uint32_t live = uint32_t(__popcll(__ballot(1)));
+  const uint32_t need = wait ? wait : 1u;  // a round that ends, ends with a lane ready
   for (;;) {
-    while (node < kRefDone) {
-      const float4* np = reinterpret_cast<const float4*>(s.nodes + node);
+    const bool down = node < kRefDone;
+    const uint64_t m = __ballot(down);
+    if (m == 0 || live - uint32_t(__popcll(m)) + extra >= need) break;
+    if (down) {
+      const float4* np = reinterpret_cast<const float4*>(nodes + node);
       const float4 a = np[0], b = np[1], c = np[2];
       const uint4 d = reinterpret_cast<const uint4*>(np)[3];
-      if (COUNT) cnt.inner++;
-      const float lim = ANY ? tmax : hit.t;
+      if (COUNT) ++inner;
       float d0, d1;
-      if (FAST) {
+      if (FASTONLY || fast) {
         d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
         d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
       } else {
@@ -164,7 +180,23 @@ __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const
         node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
       }
     }
+  }
+}
+
+// Walk the tree for one ray. ANY: stop at the first accepted triangle (returns true).
+// `stk` points at this lane's column of the block's LDS stack (stride kTraceBlock).
+template <bool ANY, bool COUNT, int STACK, bool FAST>
+__device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const Ray3& r, float tmax, Closest& hit,
+                                         Counters& cnt, uint32_t wait) {
+  RayRcp rq{};
+  if (FAST) rq = RayRcp{recip64(r.dx), recip64(r.dy), recip64(r.dz)};
+  uint32_t node = s.root;
+  int sp = 0;
+  if (!ANY) hit.t = tmax;
+  for (;;) {
+    descendRound<STACK, FAST, COUNT>(s.nodes, stk, r, rq, FAST, ANY ? tmax : hit.t, node, sp, wait, 0u, cnt.inner);
     if (node == kRefDone) break;
+    if (node < kRefDone) continue;  // still descending: next round
     if (COUNT) cnt.leaves++;
     if (node != kRefEmpty) {
       uint32_t slot = node & ~kRefLeaf;
@@ -187,6 +219,7 @@ __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const
       }
     }
     node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
+    if (node == kRefDone) break;
   }
   return false;
 }
@@ -200,9 +233,9 @@ __device__ __forceinline__ bool fastLane(const DevScene& s, const Ray3& r, float
 // One traversal routine per wave: the fast slab division when every active lane qualifies.
 template <bool ANY, bool COUNT, int STACK>
 __device__ __forceinline__ bool traverseAuto(const DevScene& s, uint32_t* stk, const Ray3& r, float tmax,
-                                             Closest& hit, Counters& cnt) {
-  if (__all(fastLane(s, r, tmax) ? 1 : 0)) return traverse<ANY, COUNT, STACK, true>(s, stk, r, tmax, hit, cnt);
-  return traverse<ANY, COUNT, STACK, false>(s, stk, r, tmax, hit, cnt);
+                                             Closest& hit, Counters& cnt, uint32_t wait = kGridWait) {
+  if (__all(fastLane(s, r, tmax) ? 1 : 0)) return traverse<ANY, COUNT, STACK, true>(s, stk, r, tmax, hit, cnt, wait);
+  return traverse<ANY, COUNT, STACK, false>(s, stk, r, tmax, hit, cnt, wait);
 }
 
 __device__ __forceinline__ Ray3 loadRay(const float4* rays, uint32_t i, float& tmax) {
@@ -217,7 +250,7 @@ template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s, const float4* __restrict__ rays, uint32_t n,
                                                              float4* __restrict__ hits, uint32_t* __restrict__ leafNid,
                                                              uint32_t* __restrict__ counters,
-                                                             const uint32_t* __restrict__ perm) {
+                                                             const uint32_t* __restrict__ perm, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
   const uint32_t lanePos = xcdBlock() * kTraceBlock + threadIdx.x;
   if (lanePos >= n) return;
@@ -226,7 +259,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s, const f
   const Ray3 r = loadRay(rays, i, tmax);
   Closest h{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
   Counters c;
-  traverseAuto<false, COUNT, STACK>(s, lds + threadIdx.x, r, tmax, h, c);
+  traverseAuto<false, COUNT, STACK>(s, lds + threadIdx.x, r, tmax, h, c, wait);
   hits[i] = make_float4(__uint_as_float(h.tri), h.t, h.u, h.v);
   if (leafNid) leafNid[i] = h.nid;
   if (COUNT) {
@@ -239,7 +272,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s, const f
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s, const float4* __restrict__ rays, uint32_t n,
                                                          uint32_t* __restrict__ occluded, uint32_t* __restrict__ counters,
-                                                         const uint32_t* __restrict__ perm) {
+                                                         const uint32_t* __restrict__ perm, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
   const uint32_t lanePos = xcdBlock() * kTraceBlock + threadIdx.x;
   if (lanePos >= n) return;
@@ -248,7 +281,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s, const float
   const Ray3 r = loadRay(rays, i, tmax);
   Closest h{};
   Counters c;
-  const bool occ = traverseAuto<true, COUNT, STACK>(s, lds + threadIdx.x, r, tmax, h, c);
+  const bool occ = traverseAuto<true, COUNT, STACK>(s, lds + threadIdx.x, r, tmax, h, c, wait);
   occluded[i] = occ ? 1u : 0u;
   if (COUNT) {
     counters[3 * i] = c.inner;
@@ -268,46 +301,43 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s, const float
 // Either way a wave's lanes hold nearby rays of the buffer.
 constexpr uint32_t kRefillChunk = 256;
 
-// Descend from `node` to the next leaf (or to the end of the traversal) for one lane, testing both
-// child boxes of every inner node as traverse() does. FASTONLY: every active lane of the wave takes
-// the reciprocal slab form (decided per wave), so no per-lane branch is needed.
-template <int STACK, bool FASTONLY>
-__device__ __forceinline__ void descend(const DevNode* __restrict__ nodes, uint32_t* stk, const Ray3& r,
-                                        const RayRcp& rq, bool fast, float lim, uint32_t& node, int& sp) {
-  while (node < kRefDone) {
-    const float4* np = reinterpret_cast<const float4*>(nodes + node);
-    const float4 a = np[0], b = np[1], c = np[2];
-    const uint4 d = reinterpret_cast<const uint4*>(np)[3];
-    float d0, d1;
-    if (FASTONLY || fast) {
-      d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
-      d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
-    } else {
-      d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, lim);
-      d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+// Chunk queues, one per XCD. Blocks are handed to the 8 XCDs round-robin (block b runs on XCD b % 8), and
+// each XCD has its own L2. A wave first takes chunks from its own XCD's share of the rays, a contiguous
+// eighth, and once that share is used up moves on to the next XCD's share, and so on; so while there is
+// work an XCD traces the rays of its own share: in a batch, the items (scenes) of that share; after a ray
+// sort, one region of space. `counters` holds kXcds words, zero at launch; `q` and `tried` are the wave's
+// (uniform) position in this walk. Returns false once every share is used up.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ bool takeChunk(uint32_t* __restrict__ counters, uint32_t n, uint32_t& q, uint32_t& tried,
+                                          uint32_t& next, uint32_t& end) {
+  const uint32_t lane = threadIdx.x & 63u;
+  while (tried < kXcds) {
+    const uint32_t lo = uint32_t(uint64_t(n) * q / kXcds), hi = uint32_t(uint64_t(n) * (q + 1u) / kXcds);
+    uint32_t got = 0;
+    if (lane == 0) got = atomicAdd(counters + q, kRefillChunk);
+    got = __builtin_amdgcn_readfirstlane(__shfl(got, 0, 64));
+    if (uint64_t(lo) + got < hi) {
+      next = lo + got;
+      end = uint32_t(min(uint64_t(hi), uint64_t(next) + kRefillChunk));
+      return true;
     }
-    const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
-    if (h0 && h1) {
-      const bool n1 = d1 < d0;
-      if (sp < STACK) stk[sp * kTraceBlock] = n1 ? d.x : d.y;
-      ++sp;
-      node = n1 ? d.y : d.x;
-    } else if (h0 || h1) {
-      node = h0 ? d.x : d.y;
-    } else {
-      node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
-    }
+    q = (q + 1u) % kXcds;
+    ++tried;
   }
+  return false;
 }
 
 template <int STACK, bool ANY, bool RANGE>
 __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, const float4* __restrict__ rays,
                                             uint32_t n, void* __restrict__ out, uint32_t* __restrict__ leafNid,
-                                            uint32_t* __restrict__ counter, const uint32_t* __restrict__ perm) {
+                                            uint32_t* __restrict__ counter, const uint32_t* __restrict__ perm,
+                                            uint32_t wait) {
   const uint32_t lane = threadIdx.x & 63u;
   bool active = false, fast = false, exhausted = false;
   uint32_t rid = 0, node = kRefDone;
   uint32_t next = 0, end = 0;  // the wave's current source of rays [next, end)
+  uint32_t q = blockIdx.x % kXcds, tried = 0;
   if (RANGE) {
     const uint32_t waves = gridDim.x * (kTraceBlock / 64u);
     const uint32_t wave = xcdBlock() * (kTraceBlock / 64u) + (threadIdx.x >> 6);
@@ -323,17 +353,7 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
   for (;;) {
     const uint64_t idle = __ballot(!active);
     if (idle && !exhausted) {
-      if (!RANGE && next >= end) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(counter, kRefillChunk);
-        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
-        if (base >= n) {
-          exhausted = true;
-        } else {
-          next = base;
-          end = uint32_t(min(uint64_t(n), uint64_t(base) + kRefillChunk));
-        }
-      }
+      if (!RANGE && next >= end && !takeChunk(counter, n, q, tried, next, end)) exhausted = true;
       if (next < end) {
         const uint32_t want = uint32_t(__popcll(idle));
         const uint32_t base = next;
@@ -355,12 +375,15 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
       }
     }
     if (!__any(active)) break;
+    // free lanes count as ready while rays remain: they take new rays after this round
+    const uint32_t extra = exhausted ? 0u : uint32_t(__popcll(__ballot(!active)));
     if (!active) continue;
     bool accepted = false;
     const float lim = ANY ? tmax : hit.t;
-    if (__all(fast ? 1 : 0)) descend<STACK, true>(s.nodes, stk, r, rq, fast, lim, node, sp);
-    else descend<STACK, false>(s.nodes, stk, r, rq, fast, lim, node, sp);
-    if (node != kRefDone) {
+    uint32_t unused = 0;
+    if (__all(fast ? 1 : 0)) descendRound<STACK, true, false>(s.nodes, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
+    else descendRound<STACK, false, false>(s.nodes, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
+    if (node > kRefDone) {
       if (node != kRefEmpty) {
         uint32_t slot = node & ~kRefLeaf;
         for (;;) {
@@ -402,18 +425,18 @@ __global__ __launch_bounds__(kTraceBlock) void kTracePersistent(DevScene s, cons
                                                                 uint32_t n, void* __restrict__ out,
                                                                 uint32_t* __restrict__ leafNid,
                                                                 uint32_t* __restrict__ counter,
-                                                                const uint32_t* __restrict__ perm) {
+                                                                const uint32_t* __restrict__ perm, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
-  refillTrace<STACK, ANY, false>(s, lds + threadIdx.x, rays, n, out, leafNid, counter, perm);
+  refillTrace<STACK, ANY, false>(s, lds + threadIdx.x, rays, n, out, leafNid, counter, perm, wait);
 }
 
 template <int STACK, bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void kTraceWaveRange(DevScene s, const float4* __restrict__ rays,
                                                                uint32_t n, void* __restrict__ out,
                                                                uint32_t* __restrict__ leafNid,
-                                                               const uint32_t* __restrict__ perm) {
+                                                               const uint32_t* __restrict__ perm, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
-  refillTrace<STACK, ANY, true>(s, lds + threadIdx.x, rays, n, out, leafNid, nullptr, perm);
+  refillTrace<STACK, ANY, true>(s, lds + threadIdx.x, rays, n, out, leafNid, nullptr, perm, wait);
 }
 
 // ---- ray sort keys -------------------------------------------------------------------------------
@@ -712,15 +735,17 @@ struct BatchItemDev {
 };
 static_assert(sizeof(BatchItemDev) == 48, "BatchItemDev is 48 B");
 
+// Zero the kXcds chunk-queue words of a refilling launch (the chunked batch kernel uses the first).
 __global__ void kResetCounter(uint32_t* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+  if (threadIdx.x < kXcds && blockIdx.x == 0) counter[threadIdx.x] = 0u;
 }
 
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* __restrict__ scenes,
                                                                 const BatchItemDev* __restrict__ items,
                                                                 uint32_t numItems, uint32_t totalChunks,
-                                                                uint32_t chunkRays, uint32_t* __restrict__ counter) {
+                                                                uint32_t chunkRays, uint32_t* __restrict__ counter,
+                                                                uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t* stk = lds + threadIdx.x;
@@ -746,12 +771,12 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* 
         Counters c;
         if (it.flags == 1u) {
           Closest h{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
-          traverseAuto<false, false, STACK>(s, stk, r, tmax, h, c);
+          traverseAuto<false, false, STACK>(s, stk, r, tmax, h, c, wait);
           reinterpret_cast<float4*>(it.out)[i] = make_float4(__uint_as_float(h.tri), h.t, h.u, h.v);
           if (it.nid) it.nid[i] = h.nid;
         } else {
           Closest h{};
-          const bool occ = traverseAuto<true, false, STACK>(s, stk, r, tmax, h, c);
+          const bool occ = traverseAuto<true, false, STACK>(s, stk, r, tmax, h, c, wait);
           reinterpret_cast<uint32_t*>(it.out)[i] = occ ? 1u : 0u;
         }
       }
@@ -769,13 +794,14 @@ template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __restrict__ scenes,
                                                             const BatchItemDev* __restrict__ items,
                                                             uint32_t numItems, uint32_t totalRays,
-                                                            uint32_t* __restrict__ counter) {
+                                                            uint32_t* __restrict__ counter, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
   uint32_t* stk = lds + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   bool active = false, fast = false, exhausted = false, anyRay = false;
   uint32_t rid = 0, item = 0, node = kRefDone;
   uint32_t next = 0, end = 0;  // the wave's current chunk of rays [next, end)
+  uint32_t q = blockIdx.x % kXcds, tried = 0;
   int sp = 0;
   float tmax = 0.0f;
   const DevNode* nodes = nullptr;
@@ -785,17 +811,7 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
   Closest hit{};
   for (;;) {
     const uint64_t idle = __ballot(!active);
-    if (idle && !exhausted && next >= end) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(counter, kRefillChunk);
-      base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
-      if (base >= totalRays) {
-        exhausted = true;
-      } else {
-        next = base;
-        end = uint32_t(min(uint64_t(totalRays), uint64_t(base) + kRefillChunk));
-      }
-    }
+    if (idle && !exhausted && next >= end && !takeChunk(counter, totalRays, q, tried, next, end)) exhausted = true;
     if (idle && next < end) {
       const uint32_t want = uint32_t(__popcll(idle));
       const uint32_t base = next;
@@ -827,11 +843,13 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
       }
     }
     if (!__any(active)) break;
+    const uint32_t extra = exhausted ? 0u : uint32_t(__popcll(__ballot(!active)));
     if (!active) continue;
     bool accepted = false;
-    if (__all(fast ? 1 : 0)) descend<STACK, true>(nodes, stk, r, rq, fast, hit.t, node, sp);
-    else descend<STACK, false>(nodes, stk, r, rq, fast, hit.t, node, sp);
-    if (node != kRefDone) {
+    uint32_t unused = 0;
+    if (__all(fast ? 1 : 0)) descendRound<STACK, true, false>(nodes, stk, r, rq, fast, hit.t, node, sp, wait, extra, unused);
+    else descendRound<STACK, false, false>(nodes, stk, r, rq, fast, hit.t, node, sp, wait, extra, unused);
+    if (node > kRefDone) {
       if (node != kRefEmpty) {
         uint32_t slot = node & ~kRefLeaf;
         for (;;) {

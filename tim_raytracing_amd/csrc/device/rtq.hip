@@ -27,6 +27,7 @@ struct rtq_device {
   uint32_t persistent = 0;
   uint32_t blocksPerCU = 8;
   uint32_t sortRays = 0;
+  uint32_t leafWait = 0;  // 0: rtq::kGridWait for grid dispatches, rtq::kRefillWait for refilling ones
   // ray-sort workspace (keys, values and rocPRIM's temporary storage), grown on demand and shared by
   // every sorting dispatch on this device
   std::mutex sortMu;
@@ -44,7 +45,7 @@ struct rtq_scene {
   rtq::DevScene view{};
   rtq_scene_info info{};
   std::vector<void*> allocations;
-  uint32_t* counters = nullptr;  // kCounterRing words, in `allocations`
+  uint32_t* counters = nullptr;  // kCounterRing groups of rtq::kXcds words, in `allocations`
   std::atomic<uint32_t> nextCounter{0};
 };
 
@@ -167,6 +168,10 @@ rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t valu
       if (value > 1) return fail(RTQ_ERR_INVALID_ARGUMENT, "sort is 0 or 1");
       dev->sortRays = value;
       return RTQ_OK;
+    case RTQ_OPTION_LEAF_WAIT:
+      if (value > 64) return fail(RTQ_ERR_INVALID_ARGUMENT, "leaf wait is 0 (default) or 1..64");
+      dev->leafWait = value;
+      return RTQ_OK;
   }
   return fail(RTQ_ERR_INVALID_ARGUMENT, "unknown option");
 }
@@ -177,6 +182,7 @@ rtq_status rtq_device_get_option(const rtq_device* dev, uint32_t option, uint32_
     case RTQ_OPTION_PERSISTENT: *value = dev->persistent; return RTQ_OK;
     case RTQ_OPTION_BLOCKS_PER_CU: *value = dev->blocksPerCU; return RTQ_OK;
     case RTQ_OPTION_SORT_RAYS: *value = dev->sortRays; return RTQ_OK;
+    case RTQ_OPTION_LEAF_WAIT: *value = dev->leafWait; return RTQ_OK;
   }
   return fail(RTQ_ERR_INVALID_ARGUMENT, "unknown option");
 }
@@ -321,7 +327,7 @@ rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, 
   if ((st = uploadArray(s, h.triangles, &s->view.triangles)) != RTQ_OK) return bail(st);
   {
     const uint32_t* dc = nullptr;
-    if ((st = uploadArray(s, std::vector<uint32_t>(kCounterRing, 0u), &dc)) != RTQ_OK) return bail(st);
+    if ((st = uploadArray(s, std::vector<uint32_t>(kCounterRing * rtq::kXcds, 0u), &dc)) != RTQ_OK) return bail(st);
     s->counters = const_cast<uint32_t*>(dc);
   }
   for (int k = 0; k < 3; ++k) {
@@ -412,6 +418,8 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(rays);
+  const uint32_t gridWait = s->dev->leafWait ? s->dev->leafWait : rtq::kGridWait;
+  const uint32_t refillWait = s->dev->leafWait ? s->dev->leafWait : rtq::kRefillWait;
   const uint32_t* perm = nullptr;
   std::unique_lock<std::mutex> sortLock;
   if (!counters && s->dev->sortRays) {
@@ -424,15 +432,17 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
     withStack(s->info.max_depth, [&](auto S) {
       constexpr int K = decltype(S)::value;
       if (flags == RTQ_TRACE_CLOSEST)
-        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid, perm);
+        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid, perm,
+                           refillWait);
       else
-        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, true>), pgrid, block, 0, st, s->view, r, n, out, nullptr, perm);
+        hipLaunchKernelGGL((rtq::kTraceWaveRange<K, true>), pgrid, block, 0, st, s->view, r, n, out, nullptr, perm,
+                           refillWait);
     });
     RTQ_HIP(hipGetLastError());
     return RTQ_OK;
   }
   if (!counters && s->dev->persistent) {
-    uint32_t* counter = s->counters + (s->nextCounter.fetch_add(1) % kCounterRing);
+    uint32_t* counter = s->counters + rtq::kXcds * (s->nextCounter.fetch_add(1) % kCounterRing);
     hipLaunchKernelGGL(rtq::kResetCounter, dim3(1), dim3(64), 0, st, counter);
     const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
     const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, n + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
@@ -440,10 +450,10 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
       constexpr int K = decltype(S)::value;
       if (flags == RTQ_TRACE_CLOSEST)
         hipLaunchKernelGGL((rtq::kTracePersistent<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid, counter,
-                           perm);
+                           perm, refillWait);
       else
         hipLaunchKernelGGL((rtq::kTracePersistent<K, true>), pgrid, block, 0, st, s->view, r, n, out, nullptr,
-                           counter, perm);
+                           counter, perm, refillWait);
     });
     RTQ_HIP(hipGetLastError());
     return RTQ_OK;
@@ -453,15 +463,17 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
     if (flags == RTQ_TRACE_CLOSEST) {
       float4* h = static_cast<float4*>(out);
       if (counters)
-        hipLaunchKernelGGL((rtq::kTraceClosest<K, true>), grid, block, 0, st, s->view, r, n, h, nid, counters, nullptr);
+        hipLaunchKernelGGL((rtq::kTraceClosest<K, true>), grid, block, 0, st, s->view, r, n, h, nid, counters, nullptr,
+                           gridWait);
       else
-        hipLaunchKernelGGL((rtq::kTraceClosest<K, false>), grid, block, 0, st, s->view, r, n, h, nid, nullptr, perm);
+        hipLaunchKernelGGL((rtq::kTraceClosest<K, false>), grid, block, 0, st, s->view, r, n, h, nid, nullptr, perm,
+                           gridWait);
     } else {
       uint32_t* o = static_cast<uint32_t*>(out);
       if (counters)
-        hipLaunchKernelGGL((rtq::kTraceAny<K, true>), grid, block, 0, st, s->view, r, n, o, counters, nullptr);
+        hipLaunchKernelGGL((rtq::kTraceAny<K, true>), grid, block, 0, st, s->view, r, n, o, counters, nullptr, gridWait);
       else
-        hipLaunchKernelGGL((rtq::kTraceAny<K, false>), grid, block, 0, st, s->view, r, n, o, nullptr, perm);
+        hipLaunchKernelGGL((rtq::kTraceAny<K, false>), grid, block, 0, st, s->view, r, n, o, nullptr, perm, gridWait);
     }
   });
   RTQ_HIP(hipGetLastError());

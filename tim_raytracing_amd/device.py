@@ -188,7 +188,7 @@ class Batch:
         B.check_device("rtq_batch_get_params", self.dev.lib.rtq_batch_get_params(self.handle, C.byref(p)))
         return p
 
-    def set_params(self, chunk_rays=None, blocks_per_cu=None, stack_depth=None, use_graph=None):
+    def set_params(self, chunk_rays=None, blocks_per_cu=None, stack_depth=None, use_graph=None, leaf_wait=None):
         p = self.params()
         if chunk_rays is not None:
             p.chunk_rays = chunk_rays
@@ -198,6 +198,8 @@ class Batch:
             p.stack_depth = stack_depth
         if use_graph is not None:
             p.use_graph = int(use_graph)
+        if leaf_wait is not None:
+            p.leaf_wait = leaf_wait
         B.check_device("rtq_batch_set_params", self.dev.lib.rtq_batch_set_params(self.handle, C.byref(p)))
 
     def launch(self, stream=None):

@@ -65,8 +65,9 @@ def parse():
                          "rays, persistent for incoherent bounces")
     ap.add_argument("--leaf-wait", type=int, default=0,
                     help="lanes ready for leaf tests that end a round of descent, 1..64 (0 = library default)")
-    ap.add_argument("--sort", type=int, choices=(0, 1), default=0,
-                    help="primary/incoherent: sort rays by direction octant and origin cell before tracing")
+    ap.add_argument("--sort", type=int, choices=(-1, 0, 1), default=-1,
+                    help="primary/incoherent: sort rays by direction octant and origin cell before tracing, inside "
+                         "the timed dispatch (-1 = auto: on for incoherent, off for primary)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent dispatch: blocks per CU (0 = library "
                                                                  "default)")
     return ap.parse_args()
@@ -85,12 +86,15 @@ def stack_class(depth: int) -> int:
     return 24 if depth <= 24 else (32 if depth <= 32 else 64)
 
 
-def configure_dispatch(dev, args, auto: str):
-    """Apply --dispatch / --blocks-per-cu to the device; returns a label for the output."""
+def configure_dispatch(dev, args, auto: str, auto_sort: int = 0):
+    """Apply --dispatch / --sort / --leaf-wait / --blocks-per-cu to the device; returns a label for the
+    output."""
     from tim_raytracing_amd import bindings as B
 
     if args.dispatch == "auto":
         args.dispatch = auto
+    if args.sort < 0:
+        args.sort = auto_sort
     dev.set_option(B.RTQ_OPTION_PERSISTENT, {"grid": 0, "persistent": 1, "range": 2}[args.dispatch])
     dev.set_option(B.RTQ_OPTION_SORT_RAYS, args.sort)
     dev.set_option(B.RTQ_OPTION_LEAF_WAIT, args.leaf_wait)
@@ -592,7 +596,7 @@ def bench_incoherent(args):
     b, f, W, Hh = H.workload(3, scale=args.scale, threads=threads)
     build_s = time.perf_counter() - t0
     dev = D.Device(0)
-    dispatch_cfg = configure_dispatch(dev, args, "persistent")
+    dispatch_cfg = configure_dispatch(dev, args, "persistent", auto_sort=1)
     scene = D.Scene.from_builder(dev, b)
     n = 4 << 20
     rays1 = b.surface_rays(seed=3, count=n)

@@ -2,6 +2,7 @@
 // layout built from builder blobs keeps every reference node, box, child link, leaf and triangle, lays
 // sibling inner nodes out as aligned pairs in depth-first order, and rejects broken blobs. Prints "ok".
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <set>
 #include <vector>
@@ -65,15 +66,16 @@ void checkScene(const Built& s) {
   CHECK(h.numNodes == numNodes && h.nodes.size() % 2 == 0);
   std::set<uint32_t> refNodes, usedSlots, innerSlots;
   std::vector<uint32_t> pairOrder;  // pair bases in depth-first (child 0 first) order
+  std::vector<uint32_t> slotDepth(h.nodes.size(), 0xFFFFFFFFu);
   uint32_t inner = 0, leaves = 0;
   // walk device refs and reference links side by side
-  struct Item { uint32_t ref, id; };
+  struct Item { uint32_t ref, id, depth; };
   std::vector<Item> st;
   if (numNodes == 1) {
-    st.push_back({h.root, 0});
+    st.push_back({h.root, 0, 0});
   } else {
     CHECK(h.root == 0);
-    st.push_back({0u, 0u});
+    st.push_back({0u, 0u, 0u});
   }
   while (!st.empty()) {
     const Item it = st.back();
@@ -107,6 +109,7 @@ void checkScene(const Built& s) {
     ++inner;
     CHECK(it.ref < h.nodes.size());
     CHECK(innerSlots.insert(it.ref).second);
+    slotDepth[it.ref] = it.depth;
     const rtq::DevNode& d = h.nodes[it.ref];
     CHECK(d.d[2] == it.id);
     const float* f = reinterpret_cast<const float*>(nodes + 16 * it.id);
@@ -128,13 +131,31 @@ void checkScene(const Built& s) {
       base = b;
     }
     if (base != 0xFFFFFFFFu) pairOrder.push_back(base);
-    for (int k = 1; k >= 0; --k) st.push_back({d.d[k], l + uint32_t(k)});
+    for (int k = 1; k >= 0; --k) st.push_back({d.d[k], l + uint32_t(k), it.depth + 1});
   }
   CHECK(inner == h.numInner);
   CHECK(usedSlots.size() == h.tris.size());
-  // depth-first layout: pairs appear in walk order right after the root's pair
-  for (size_t i = 0; i < pairOrder.size(); ++i) CHECK(pairOrder[i] == 2 + 2 * i);
   CHECK(h.nodes.size() == (numNodes == 1 ? 0u : 2 + 2 * pairOrder.size()));
+  // the first topSlots records hold the top levels breadth first: depths never decrease along them and
+  // no record after them is shallower than any of them
+  CHECK(h.topSlots == std::min<size_t>(rtq::kTopSlots, h.nodes.size()));
+  uint32_t topMax = 0;
+  for (uint32_t k = 0; k < h.topSlots; ++k) {
+    if (slotDepth[k] == 0xFFFFFFFFu) continue;
+    CHECK(slotDepth[k] >= topMax);
+    topMax = slotDepth[k];
+  }
+  for (size_t k = h.topSlots; k < h.nodes.size(); ++k)
+    if (slotDepth[k] != 0xFFFFFFFFu) CHECK(slotDepth[k] >= topMax);
+  if (rtq::kTopSlots == 0) {
+    // all depth first: pairs appear in walk order right after the root's pair
+    for (size_t i = 0; i < pairOrder.size(); ++i) CHECK(pairOrder[i] == 2 + 2 * i);
+  } else {
+    // pairs are dense whatever the order
+    std::set<uint32_t> bases(pairOrder.begin(), pairOrder.end());
+    CHECK(bases.size() == pairOrder.size());
+    for (size_t i = 0; i < pairOrder.size(); ++i) CHECK(bases.count(uint32_t(2 + 2 * i)) == 1);
+  }
   // every slot that holds no reachable inner node is an all-zero record
   static const rtq::DevNode zero{};
   for (size_t k = 0; k < h.nodes.size(); ++k)

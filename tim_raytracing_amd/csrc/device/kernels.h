@@ -143,12 +143,33 @@ struct Closest {
 constexpr uint32_t kGridWait = 24;    // default when each lane traces one ray
 constexpr uint32_t kRefillWait = 32;  // default when lanes refill with new rays
 
+// Top-of-tree staging: a kernel that traces one scene copies the first s.topSlots node records (the
+// top levels, laid out breadth first) into LDS once per block and reads them there. Every thread of
+// the block must call this before any of them returns.
+template <uint32_t TOP = kTopSlots>
+__device__ __forceinline__ DevScene stageTop(const DevScene& s) {
+  DevScene v = s;
+  if constexpr (TOP > 0) {
+    __shared__ DevNode top[TOP];
+    const uint32_t words = s.topSlots * 4u;
+    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x)
+      reinterpret_cast<float4*>(top)[k] = reinterpret_cast<const float4*>(s.nodes)[k];
+    __syncthreads();
+    v.top = top;
+  } else {
+    v.topSlots = 0;
+    v.top = nullptr;
+  }
+  return v;
+}
+
 // One round of descent for the calling lanes. FASTONLY: every calling lane takes the reciprocal slab
 // form (decided per wave), so no per-lane branch is needed; otherwise `fast` chooses per lane.
 template <int STACK, bool FASTONLY, bool COUNT>
-__device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, uint32_t* stk, const Ray3& r,
-                                             const RayRcp& rq, bool fast, float lim, uint32_t& node, int& sp,
-                                             uint32_t wait, uint32_t extra, uint32_t& inner) {
+__device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, const DevNode* top, uint32_t topSlots,
+                                             uint32_t* stk, const Ray3& r, const RayRcp& rq, bool fast, float lim,
+                                             uint32_t& node, int& sp, uint32_t wait, uint32_t extra,
+                                             uint32_t& inner) {
   const uint32_t live = uint32_t(__popcll(__ballot(1)));
   const uint32_t need = wait ? wait : 1u;  // a round that ends, ends with a lane ready
   for (;;) {
@@ -156,7 +177,11 @@ __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, 
     const uint64_t m = __ballot(down);
     if (m == 0 || live - uint32_t(__popcll(m)) + extra >= need) break;
     if (down) {
-      const float4* np = reinterpret_cast<const float4*>(nodes + node);
+      const DevNode* src = nodes;
+      if constexpr (kTopSlots > 0) {
+        if (node < topSlots) src = top;  // the block's LDS copy of the top levels
+      }
+      const float4* np = reinterpret_cast<const float4*>(src + node);
       const float4 a = np[0], b = np[1], c = np[2];
       const uint4 d = reinterpret_cast<const uint4*>(np)[3];
       if (COUNT) ++inner;
@@ -194,7 +219,8 @@ __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const
   int sp = 0;
   if (!ANY) hit.t = tmax;
   for (;;) {
-    descendRound<STACK, FAST, COUNT>(s.nodes, stk, r, rq, FAST, ANY ? tmax : hit.t, node, sp, wait, 0u, cnt.inner);
+    descendRound<STACK, FAST, COUNT>(s.nodes, s.top, s.topSlots, stk, r, rq, FAST, ANY ? tmax : hit.t, node, sp, wait,
+                                     0u, cnt.inner);
     if (node == kRefDone) break;
     if (node < kRefDone) continue;  // still descending: next round
     if (COUNT) cnt.leaves++;
@@ -247,11 +273,12 @@ __device__ __forceinline__ Ray3 loadRay(const float4* rays, uint32_t i, float& t
 // perm (optional): lane order -> ray index, so that neighbouring lanes trace rays a sort has put
 // together; results still land at the ray's own index.
 template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s, const float4* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s0, const float4* __restrict__ rays, uint32_t n,
                                                              float4* __restrict__ hits, uint32_t* __restrict__ leafNid,
                                                              uint32_t* __restrict__ counters,
                                                              const uint32_t* __restrict__ perm, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
+  const DevScene s = stageTop(s0);
   const uint32_t lanePos = xcdBlock() * kTraceBlock + threadIdx.x;
   if (lanePos >= n) return;
   const uint32_t i = perm ? perm[lanePos] : lanePos;
@@ -270,10 +297,11 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s, const f
 }
 
 template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s, const float4* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s0, const float4* __restrict__ rays, uint32_t n,
                                                          uint32_t* __restrict__ occluded, uint32_t* __restrict__ counters,
                                                          const uint32_t* __restrict__ perm, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
+  const DevScene s = stageTop(s0);
   const uint32_t lanePos = xcdBlock() * kTraceBlock + threadIdx.x;
   if (lanePos >= n) return;
   const uint32_t i = perm ? perm[lanePos] : lanePos;
@@ -381,8 +409,10 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
     bool accepted = false;
     const float lim = ANY ? tmax : hit.t;
     uint32_t unused = 0;
-    if (__all(fast ? 1 : 0)) descendRound<STACK, true, false>(s.nodes, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
-    else descendRound<STACK, false, false>(s.nodes, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
+    if (__all(fast ? 1 : 0))
+      descendRound<STACK, true, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
+    else
+      descendRound<STACK, false, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
     if (node > kRefDone) {
       if (node != kRefEmpty) {
         uint32_t slot = node & ~kRefLeaf;
@@ -421,21 +451,23 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
 }
 
 template <int STACK, bool ANY>
-__global__ __launch_bounds__(kTraceBlock) void kTracePersistent(DevScene s, const float4* __restrict__ rays,
+__global__ __launch_bounds__(kTraceBlock) void kTracePersistent(DevScene s0, const float4* __restrict__ rays,
                                                                 uint32_t n, void* __restrict__ out,
                                                                 uint32_t* __restrict__ leafNid,
                                                                 uint32_t* __restrict__ counter,
                                                                 const uint32_t* __restrict__ perm, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
+  const DevScene s = stageTop(s0);
   refillTrace<STACK, ANY, false>(s, lds + threadIdx.x, rays, n, out, leafNid, counter, perm, wait);
 }
 
 template <int STACK, bool ANY>
-__global__ __launch_bounds__(kTraceBlock) void kTraceWaveRange(DevScene s, const float4* __restrict__ rays,
+__global__ __launch_bounds__(kTraceBlock) void kTraceWaveRange(DevScene s0, const float4* __restrict__ rays,
                                                                uint32_t n, void* __restrict__ out,
                                                                uint32_t* __restrict__ leafNid,
                                                                const uint32_t* __restrict__ perm, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
+  const DevScene s = stageTop(s0);
   refillTrace<STACK, ANY, true>(s, lds + threadIdx.x, rays, n, out, leafNid, nullptr, perm, wait);
 }
 
@@ -494,10 +526,11 @@ __device__ __forceinline__ Ray3 cameraRay(const Frustum4& f, uint32_t W, uint32_
 // neighbouring pixels. referenceRows = 1 reproduces cameraPass.comp:46's storage row
 // 16*ceil(H/16) - gy (the generating row differs from the stored one and row 0 is never written).
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void kTraceCamera(DevScene s, Frustum4 f, uint32_t W, uint32_t H,
+__global__ __launch_bounds__(kTraceBlock) void kTraceCamera(DevScene s0, Frustum4 f, uint32_t W, uint32_t H,
                                                             uint32_t referenceRows, float4* __restrict__ hits,
                                                             uint32_t* __restrict__ leafNid) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
+  const DevScene s = stageTop(s0);
   const uint32_t tilesX = (W + 15u) / 16u;
   const uint32_t b = xcdBlock();
   const uint32_t bx = b % tilesX, by = b / tilesX;
@@ -623,11 +656,12 @@ __device__ __forceinline__ void traceRecord(const DevScene& s, uint32_t* stk, co
 // [vertexOffset, index01, index2_matId, floatBits(t)] and [shadowMask, probMask, 0, 0]. A miss keeps
 // t = TMAX and zero triangle words (the shader leaves whatever its shared memory held) and masks 0.
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void kTraceRecord(DevScene s, Frustum4 f, uint32_t W, uint32_t H,
+__global__ __launch_bounds__(kTraceBlock) void kTraceRecord(DevScene s0, Frustum4 f, uint32_t W, uint32_t H,
                                                             uint32_t referenceRows, float sunX, float sunY,
                                                             float sunZ, uint32_t shadowFlags, LpfHeader lpf,
                                                             uint32_t useLpf, uint4* __restrict__ records) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
+  const DevScene s = stageTop(s0);
   const uint32_t tilesX = (W + 15u) / 16u;
   const uint32_t b = xcdBlock();
   const uint32_t bx = b % tilesX, by = b / tilesX;
@@ -679,10 +713,11 @@ __device__ __forceinline__ void probePosition(const LpfHeader& lpf, uint32_t ind
 // the record of ray `rayId` of probe `p` lands at sample p * 64 + rayId. The probe mask is not
 // computed in this pass (the reference dispatches it without USE_LPF_MASK).
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void kTraceProbes(DevScene s, LpfHeader lpf, ProbeDirs dirs,
+__global__ __launch_bounds__(kTraceBlock) void kTraceProbes(DevScene s0, LpfHeader lpf, ProbeDirs dirs,
                                                             uint32_t numProbes, float sunX, float sunY, float sunZ,
                                                             uint32_t shadowFlags, uint4* __restrict__ records) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
+  const DevScene s = stageTop(s0);
   const uint32_t wave = xcdBlock() * (kTraceBlock / 64u) + (threadIdx.x >> 6);
   const uint32_t rayId = wave % kProbeRays;
   const uint32_t probe = (wave / kProbeRays) * 64u + (threadIdx.x & 63u);
@@ -700,12 +735,13 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceProbes(DevScene s, LpfHeade
 
 // computeShadowMask (bvhLighting.glsl:143-163) for one closest hit per lane.
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void kShadowMasks(DevScene s, const float4* __restrict__ rays,
+__global__ __launch_bounds__(kTraceBlock) void kShadowMasks(DevScene s0, const float4* __restrict__ rays,
                                                             const float4* __restrict__ hits,
                                                             const uint32_t* __restrict__ leafNid, uint32_t n,
                                                             float sunX, float sunY, float sunZ, uint32_t flags,
                                                             uint32_t* __restrict__ masks) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
+  const DevScene s = stageTop(s0);
   const uint32_t i = xcdBlock() * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   float tmax;
@@ -761,7 +797,9 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* 
       else hi = mid;
     }
     const BatchItemDev it = items[lo];
-    const DevScene s = scenes[it.scene];
+    DevScene s = scenes[it.scene];
+    s.topSlots = 0;  // the blocks of this kernel trace many scenes: nothing is staged in LDS
+    s.top = nullptr;
     const uint32_t base = (chunk - it.chunkBegin) * chunkRays;
     for (uint32_t g = 0; g < chunkRays; g += 64u) {
       const uint32_t i = base + g + lane;
@@ -847,8 +885,11 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
     if (!active) continue;
     bool accepted = false;
     uint32_t unused = 0;
-    if (__all(fast ? 1 : 0)) descendRound<STACK, true, false>(nodes, stk, r, rq, fast, hit.t, node, sp, wait, extra, unused);
-    else descendRound<STACK, false, false>(nodes, stk, r, rq, fast, hit.t, node, sp, wait, extra, unused);
+    // lanes of one block trace different scenes here, so nothing is staged in LDS
+    if (__all(fast ? 1 : 0))
+      descendRound<STACK, true, false>(nodes, nullptr, 0u, stk, r, rq, fast, hit.t, node, sp, wait, extra, unused);
+    else
+      descendRound<STACK, false, false>(nodes, nullptr, 0u, stk, r, rq, fast, hit.t, node, sp, wait, extra, unused);
     if (node > kRefDone) {
       if (node != kRefEmpty) {
         uint32_t slot = node & ~kRefLeaf;

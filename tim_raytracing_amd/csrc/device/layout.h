@@ -21,6 +21,13 @@
 
 namespace rtq {
 
+// Inner-node records at the start of the layout that hold the top levels of the tree (breadth first)
+// and that single-scene kernels copy into LDS once per block. 0 disables the staging.
+#ifndef RTQ_TOP_SLOTS
+#define RTQ_TOP_SLOTS 0
+#endif
+constexpr uint32_t kTopSlots = RTQ_TOP_SLOTS;
+
 constexpr uint32_t kRefLeaf = 0x80000000u;
 constexpr uint32_t kRefEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kRefDone = 0x7FFFFFFFu;  // traversal finished (never a valid inner index)
@@ -50,6 +57,8 @@ struct DevScene {
   uint32_t numLights;
   uint32_t fastBoxes;        // 1 if every inner-node box bound qualifies for the fast slab division
   const uint32_t* triangles; // the blob's Triangle records (3 words each), indexed by triangle id
+  uint32_t topSlots;         // leading node records a kernel may read from its LDS copy (<= kTopSlots)
+  const DevNode* top;        // that LDS copy, set by the kernel; nullptr outside a staging kernel
 };
 
 // Light-probe field placement as cameraPass.comp builds LightProbFieldHeader (cameraPass.comp:62-66).

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <unordered_map>
 #include <vector>
 
@@ -22,6 +23,7 @@ struct HostScene {
   std::vector<uint32_t> lightKinds;  // PackedLight.iparam per light
   std::vector<uint32_t> triangles;   // the blob's Triangle records, 3 words each
   uint32_t root = kRefEmpty;         // root ref
+  uint32_t topSlots = 0;             // leading node records holding the top levels, breadth first
   uint32_t numNodes = 0, numInner = 0, leafWords = 0, numLights = 0, numTris = 0, maxDepth = 0;
   bool fastBoxes = true;             // every box bound qualifies for the reciprocal slab form
 };
@@ -95,9 +97,10 @@ inline rtq_status buildHostScene(const uint8_t* blob, size_t bytes, const rtq_ra
     }
   }
 
-  // Device order: depth first from the root, the two children of an inner node side by side in one
-  // 128-byte pair (a pop then usually finds its node in the line its sibling already brought into the
-  // cache), and the triangle slots of the leaves in the same depth-first order. Leaf children need no
+  // Device order: the two children of an inner node side by side in one 128-byte pair (a pop then
+  // usually finds its node in the line its sibling already brought into the cache); the first kTopSlots
+  // records hold the top levels breadth first (kernels copy them into LDS), the rest follow depth first,
+  // and the triangle slots of the leaves in the order their parents are laid out. Leaf children need no
   // record; their half of a pair stays empty. Traversal order is fixed by the child refs alone, so the
   // layout changes no result.
   std::vector<uint32_t> innerIndex(numNodes, 0xFFFFFFFFu);
@@ -143,10 +146,13 @@ inline rtq_status buildHostScene(const uint8_t* blob, size_t bytes, const rtq_ra
   } else {
     innerIndex[0] = 0;
     slotOwner = {0u, 0xFFFFFFFFu};  // the root alone in the first pair
-    std::vector<uint32_t> todo{0u};
-    while (!todo.empty()) {
-      const uint32_t id = todo.back();
-      todo.pop_back();
+    std::deque<uint32_t> wide{0u};  // breadth first while the top records fill
+    std::vector<uint32_t> todo;     // depth first afterwards
+    while (!wide.empty() || !todo.empty()) {
+      const bool fromWide = !wide.empty();
+      const uint32_t id = fromWide ? wide.front() : todo.back();
+      if (fromWide) wide.pop_front();
+      else todo.pop_back();
       const uint32_t l = nodes[16 * id + 2] & 0x3FFFFFFFu;
       const uint32_t c[2] = {l, l + 1u};
       if (!isLeaf[c[0]] || !isLeaf[c[1]]) {
@@ -161,10 +167,14 @@ inline rtq_status buildHostScene(const uint8_t* blob, size_t bytes, const rtq_ra
       }
       for (int k = 0; k < 2; ++k)
         if (isLeaf[c[k]] && (st = addLeaf(c[k])) != RTQ_OK) return st;
+      const bool wideChildren = fromWide && slotOwner.size() < kTopSlots;
+      for (int k = 0; k < 2; ++k)
+        if (!isLeaf[c[k]] && wideChildren) wide.push_back(c[k]);
       for (int k = 1; k >= 0; --k)  // walk child 0's subtree first
-        if (!isLeaf[c[k]]) todo.push_back(c[k]);
+        if (!isLeaf[c[k]] && !wideChildren) todo.push_back(c[k]);
     }
     h.root = 0;
+    h.topSlots = uint32_t(std::min<size_t>(kTopSlots, slotOwner.size()));
   }
   auto ref = [&](uint32_t id) { return isLeaf[id] ? leafSlot[id] : innerIndex[id]; };
   h.nodes.assign(slotOwner.size(), DevNode{});

@@ -115,9 +115,9 @@ rtq_status rtq_device_destroy(rtq_device* dev);
  * RTQ_OPTION_BLOCKS_PER_CU: persistent blocks of 128 lanes per compute unit (1..16, default 8). */
 #define RTQ_OPTION_PERSISTENT 1u
 #define RTQ_OPTION_BLOCKS_PER_CU 2u
-/* RTQ_OPTION_SORT_RAYS: 1 = rtq_dispatch_rays first sorts the rays by direction octant and a Morton
- * code of the origin over the scene box (rocPRIM radix sort) and traces them in that order, so that
- * a wave's lanes hold rays that start close together and head the same way; results still land at
+/* RTQ_OPTION_SORT_RAYS: 1 = rtq_dispatch_rays first sorts the rays by a Morton code of the origin over
+ * the scene box, then direction octant (rocPRIM radix sort) and traces them in that order, so that a
+ * wave's lanes hold rays that start close together and head the same way; results still land at
  * each ray's own index and are identical. The sort workspace belongs to the device and grows on
  * demand: sorting dispatches on one device must be issued on one stream. Default 0. */
 #define RTQ_OPTION_SORT_RAYS 3u

@@ -472,9 +472,10 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceWaveRange(DevScene s0, cons
 }
 
 // ---- ray sort keys -------------------------------------------------------------------------------
-// Key of ray i: direction octant in bits 27-29 above a 27-bit Morton code of the origin quantised to
-// 512 cells per axis over the scene box (out-of-box and NaN coordinates clamp to the border cells).
-// Sorting by it puts rays that start close together and head the same way on neighbouring lanes.
+// Key of ray i: a 27-bit Morton code of the origin quantised to 512 cells per axis over the scene box
+// (out-of-box and NaN coordinates clamp to the border cells) in bits 3-29, the direction octant in bits
+// 0-2. Sorting by it puts rays that start close together on neighbouring lanes, those of one cell
+// grouped by direction. (The octant above the Morton code traced the soup's bounces 7 % slower.)
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 9 bits -> every third of 27 bits
   v &= 0x1FFu;
   v = (v | (v << 16)) & 0x030000FFu;
@@ -494,7 +495,7 @@ __global__ __launch_bounds__(256) void kRayKeys(const float4* __restrict__ rays,
 #pragma unroll
   for (int k = 0; k < 3; ++k) c[k] = q[k] >= 0.0f ? (q[k] < 511.0f ? uint32_t(q[k]) : 511u) : 0u;
   const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
-  keys[i] = (oct << 27) | (spread3(c[0]) << 2) | (spread3(c[1]) << 1) | spread3(c[2]);
+  keys[i] = (((spread3(c[0]) << 2) | (spread3(c[1]) << 1) | spread3(c[2])) << 3) | oct;
   vals[i] = i;
 }
 

@@ -319,12 +319,13 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s0, const floa
 }
 
 // ---- persistent traversal with lane refill ------------------------------------------------
-// A fixed grid of waves; each lane traces one ray at a time. After every descent-to-leaf step the
-// lanes whose ray has finished take new rays, so lanes no longer idle while the slowest ray of their
-// wave finishes. Each ray still follows the reference visit order exactly; only the assignment of
-// rays to lanes changes. Two sources of rays:
-//   RANGE = false: chunks of kRefillChunk consecutive rays taken from a device counter when the
-//     wave's current chunk runs out (one atomic per chunk, so waves balance dynamically);
+// A fixed grid of waves; each lane traces one ray at a time. After every round of descent and leaf
+// tests the lanes whose ray has finished take new rays, so lanes no longer idle while the slowest ray
+// of their wave finishes. Each ray still follows the reference visit order exactly; only the
+// assignment of rays to lanes changes. Two sources of rays:
+//   RANGE = false: chunks of kRefillChunk consecutive rays taken from the per-XCD chunk queues
+//     (takeChunk) when the wave's current chunk runs out (one atomic per chunk, so waves balance
+//     dynamically);
 //   RANGE = true: one fixed contiguous range per wave (no atomics).
 // Either way a wave's lanes hold nearby rays of the buffer.
 constexpr uint32_t kRefillChunk = 256;
@@ -824,11 +825,12 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* 
 }
 
 // Lane-refilling form of the batch: lanes take single rays from the concatenation of all items (the
-// wave draws chunks of kRefillChunk rays from a device counter; a binary search over rayBegin maps a
-// ray to its item), so a lane that finishes early picks up the next ray, whatever its item, scene or
-// mode, instead of idling until its wave's slowest ray is done. The scene pointers and the
-// closest/any mode are per lane. Each ray still follows the reference visit order exactly; every wave
-// reaches the exit test once the counter passes totalRays.
+// wave draws chunks of kRefillChunk rays from the per-XCD chunk queues, so an XCD works through the
+// items of its own share first; a binary search over rayBegin maps a ray to its item), so a lane that
+// finishes early picks up the next ray, whatever its item, scene or mode, instead of idling until its
+// wave's slowest ray is done. The scene pointers and the closest/any mode are per lane. Each ray still
+// follows the reference visit order exactly; every wave reaches the exit test once every share is used
+// up.
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __restrict__ scenes,
                                                             const BatchItemDev* __restrict__ items,

@@ -253,6 +253,19 @@ def main():
     camera_ms = timed(lambda: B.check_device("rtq_trace_camera", dev.lib.rtq_trace_camera(
         scene.handle, C.byref(frustum), W, Hh, 0, d_hits.ptr, d_nid.ptr, sptr)), max(1, args.warmup // 2), args.steps)
 
+    # the full camera-pass record (cameraPass.comp TRACING_STEP): closest hit, shadow mask (sun + the hit
+    # leaf's lights) and probe mask (8 any-hit segments) per pixel, 32 B per pixel
+    sun = np.array([0.3, 0.2, -1.0], np.float32)
+    sun /= np.linalg.norm(sun)
+    smn, smx = builder.scene_box()
+    lpf = B.Lpf((C.c_float * 3)(*[float(x) for x in smn]), (C.c_float * 3)(*[float(x) for x in smx]),
+                (C.c_uint32 * 3)(16, 16, 16))
+    d_records = dev.alloc(32 * n)
+    record_ms = timed(lambda: B.check_device("rtq_trace_camera_record", dev.lib.rtq_trace_camera_record(
+        scene.handle, C.byref(frustum), W, Hh, 0, sun.ctypes.data, B.RTQ_SHADOW_SUN, C.byref(lpf), d_records.ptr,
+        sptr)), max(1, args.warmup // 2), args.steps)
+    del d_records
+
     # the same rays in the other buffer order, for comparison (same kernel, same per-ray work)
     d_rays.write(rays_other)
     other_ms = timed(closest, max(1, args.warmup // 2), args.steps)
@@ -308,6 +321,9 @@ def main():
         },
         "camera_pass": {"kernel": "kTraceCamera", "ms_per_step": round(camera_ms, 4),
                         "mrays_s": round(n / (camera_ms * 1e-3) / 1e6, 2)},
+        "record_pass": {"kernel": "kTraceRecord", "ms_per_step": round(record_ms, 4),
+                        "mpixels_s": round(n / (record_ms * 1e-3) / 1e6, 2),
+                        "per_pixel": "closest hit + sun and leaf-light shadow rays + 8 probe segments (16^3 field)"},
         "other_ray_order": {"ray_order": other_order, "ms_per_step": round(other_ms, 4),
                             "mrays_s": round(n / (other_ms * 1e-3) / 1e6, 2)},
         "host": {"threads": threads, "bvh_build_s": round(build_s, 2), "setup_s": round(setup_s, 2)},

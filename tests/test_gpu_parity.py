@@ -229,6 +229,28 @@ def test_soup_cut_bounce_rays():
     assert np.array_equal(ga, oa)
 
 
+def test_soup_shadow_masks_many_lights():
+    """Leaf light lists with many entries, and light indices past the 31 mask bits (skipped, as the shader's
+    shift is undefined there)."""
+    _print("case: soup cut with 341 point lights, shadow masks of bounce-1 hits, with and without the sun")
+    s = G.scene(3, 1 / 48)
+    b = s["builder"]
+    rays = b.surface_rays(seed=13, count=200_000)
+    g = s["gpu"].trace_closest(rays)
+    o = s["orc"].trace_closest(rays, threads=16)
+    G.assert_closest_equal(g, o, "soup")
+    sun = np.array([-0.2, 0.4, -0.89], np.float32)
+    sun /= np.linalg.norm(sun)
+    for flags in (0, B.RTQ_SHADOW_SUN):
+        gm = s["gpu"].shadow(rays, g["hits"], g["nid"], sun, flags)
+        om = s["orc"].shadow_masks(rays, o["t"], o["nid"], sun, flags, threads=16)
+        assert np.array_equal(gm, om), f"flags {flags}: {np.count_nonzero(gm != om)} masks differ"
+    hit = o["tri_id"] != B.RTQ_NO_HIT
+    light_bits = om[hit] >> 1
+    assert (light_bits != 0).any()                                   # some hits see lights
+    assert (np.bitwise_count(light_bits) >= 2).any()                 # and several per leaf
+
+
 def test_tile_light_culling_4096_lights():
     _print("case: 4096 point lights culled into 16x16 tiles at 1920x1080")
     b = H.BVHBuilder()

@@ -172,8 +172,9 @@ def cpu_baseline(blob, ranges, pos, rays, threads: int, target_s: float):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of "
-                                   "this build", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark measures one GPU; --gpus above 1 is not supported (rays "
+                                   "are independent, so a multi-GPU run would split them into contiguous "
+                                   "ranges over replicated scenes)", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.mode == "batch":
         return bench_batch(args)

@@ -335,20 +335,21 @@ constexpr uint32_t kRefillChunk = 256;
 // eighth, and once that share is used up moves on to the next XCD's share, and so on; so while there is
 // work an XCD traces the rays of its own share: in a batch, the items (scenes) of that share; after a ray
 // sort, one region of space. `counters` holds kXcds words, zero at launch; `q` and `tried` are the wave's
-// (uniform) position in this walk. Returns false once every share is used up.
+// (uniform) position in this walk; [next, end) receives up to `grab` consecutive units (rays, or whole
+// chunks of a batch). Returns false once every share is used up.
 constexpr uint32_t kXcds = 8;
 
-__device__ __forceinline__ bool takeChunk(uint32_t* __restrict__ counters, uint32_t n, uint32_t& q, uint32_t& tried,
-                                          uint32_t& next, uint32_t& end) {
+__device__ __forceinline__ bool takeChunk(uint32_t* __restrict__ counters, uint32_t n, uint32_t grab, uint32_t& q,
+                                          uint32_t& tried, uint32_t& next, uint32_t& end) {
   const uint32_t lane = threadIdx.x & 63u;
   while (tried < kXcds) {
     const uint32_t lo = uint32_t(uint64_t(n) * q / kXcds), hi = uint32_t(uint64_t(n) * (q + 1u) / kXcds);
     uint32_t got = 0;
-    if (lane == 0) got = atomicAdd(counters + q, kRefillChunk);
+    if (lane == 0) got = atomicAdd(counters + q, grab);
     got = __builtin_amdgcn_readfirstlane(__shfl(got, 0, 64));
     if (uint64_t(lo) + got < hi) {
       next = lo + got;
-      end = uint32_t(min(uint64_t(hi), uint64_t(next) + kRefillChunk));
+      end = uint32_t(min(uint64_t(hi), uint64_t(next) + grab));
       return true;
     }
     q = (q + 1u) % kXcds;
@@ -382,7 +383,7 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
   for (;;) {
     const uint64_t idle = __ballot(!active);
     if (idle && !exhausted) {
-      if (!RANGE && next >= end && !takeChunk(counter, n, q, tried, next, end)) exhausted = true;
+      if (!RANGE && next >= end && !takeChunk(counter, n, kRefillChunk, q, tried, next, end)) exhausted = true;
       if (next < end) {
         const uint32_t want = uint32_t(__popcll(idle));
         const uint32_t base = next;
@@ -761,8 +762,9 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowMasks(DevScene s0, const f
 
 // ---- batched form: many scenes and ray sets in one persistent launch --------------------------
 // Work is split into chunks of `chunkRays` consecutive rays of one item; items are sorted by
-// chunkBegin. Each wave takes the next chunk from a device counter until all chunks are gone,
-// so every wave reaches the exit test and the grid drains.
+// chunkBegin. Each wave takes the next chunk from the per-XCD chunk queues (so an XCD works through
+// the items of its own share first) until all chunks are gone, so every wave reaches the exit test and
+// the grid drains.
 struct BatchItemDev {
   uint32_t scene, flags, numRays, chunkBegin;
   const float4* rays;
@@ -787,11 +789,10 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* 
   __shared__ uint32_t lds[STACK * kTraceBlock];
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t* stk = lds + threadIdx.x;
+  uint32_t q = blockIdx.x % kXcds, tried = 0;
   for (;;) {
-    uint32_t chunk = 0;
-    if (lane == 0) chunk = atomicAdd(counter, 1u);
-    chunk = __builtin_amdgcn_readfirstlane(__shfl(chunk, 0, 64));
-    if (chunk >= totalChunks) break;
+    uint32_t chunk = 0, chunkEnd = 0;  // one chunk at a time from the per-XCD queues
+    if (!takeChunk(counter, totalChunks, 1u, q, tried, chunk, chunkEnd)) break;
     uint32_t lo = 0, hi = numItems;
     while (hi - lo > 1u) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -852,7 +853,8 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
   Closest hit{};
   for (;;) {
     const uint64_t idle = __ballot(!active);
-    if (idle && !exhausted && next >= end && !takeChunk(counter, totalRays, q, tried, next, end)) exhausted = true;
+    if (idle && !exhausted && next >= end && !takeChunk(counter, totalRays, kRefillChunk, q, tried, next, end))
+      exhausted = true;
     if (idle && next < end) {
       const uint32_t want = uint32_t(__popcll(idle));
       const uint32_t base = next;

@@ -525,3 +525,47 @@ def test_tile_planes_contain_tile_centre_rays():
         q = cam + (other - cam) * 37.0
         assert np.any(pl[:, :3] @ q + pl[:, 3] < 0)
         assert np.allclose(np.linalg.norm(pl[:, :3], axis=1), 1, atol=1e-6)
+
+
+# ---------------------------------------------------------------- camera model (SimpleCamera, cameraPass.comp:50-54)
+def test_camera_model_closed_form():
+    """The frustum and the camera-pass rays against what the projection fixes: a vertical field of view
+    `fov_y` and a horizontal one from the aspect ratio, pixel (x, y) aimed at the pixel's corner
+    c00 + stepW*x + stepH*y (no half-pixel offset), the frame centre along the view direction. A swapped
+    axis, a horizontal fov, a half-pixel shift or a wrong aspect fails one of these."""
+    W, Hh = 64, 36
+    fov = H.REFERENCE_FOV
+    aspect = W / Hh
+    pos = np.array([1.0, -2.0, 0.5])
+    f = H.camera_frustum(tuple(pos), (1, 0, 0), (0, 0, 1), fov, aspect)
+    fr = oracle.frustum_array(f).astype(np.float64).reshape(4, 3)
+    cam, c00, c10, c01 = fr
+    assert np.allclose(cam, pos)
+    th, tw = np.tan(fov / 2), aspect * np.tan(fov / 2)
+    # the corners sit on one plane across the view axis (+x), c10 - c00 spans the width (along y),
+    # c01 - c00 the height (along z), at the field-of-view angles
+    dist = c00[0] - cam[0]
+    assert dist > 0 and np.isclose(c10[0] - cam[0], dist) and np.isclose(c01[0] - cam[0], dist)
+    w_vec, h_vec = c10 - c00, c01 - c00
+    assert abs(w_vec[2]) < 1e-6 * dist and abs(h_vec[1]) < 1e-6 * dist
+    assert np.isclose(abs(w_vec[1]), 2 * tw * dist, rtol=1e-5)
+    assert np.isclose(abs(h_vec[2]), 2 * th * dist, rtol=1e-5)
+    centre = c00 + 0.5 * (w_vec + h_vec)
+    assert np.allclose(centre[1:], cam[1:], atol=1e-5 * dist)
+    # rays of the pass (no jitter)
+    rays = oracle.camera_rays(f, W, Hh)
+    d = rays["dir"].astype(np.float64).reshape(Hh, W, 3)
+    assert np.allclose(rays["org"], pos.astype(np.float32))
+    assert np.allclose(np.linalg.norm(d, axis=2), 1.0, atol=1e-6)
+
+    def unit(v):
+        return v / np.linalg.norm(v)
+
+    assert np.allclose(d[0, 0], unit(c00 - cam), atol=1e-6)                # pixel (0, 0): the corner itself
+    assert np.allclose(d[Hh // 2, W // 2], [1, 0, 0], atol=1e-6)           # frame centre: the view axis
+    # pixel (0, H/2) and (W/2, 0): half the horizontal / vertical field of view off the axis
+    assert np.isclose(np.arccos(d[Hh // 2, 0] @ [1, 0, 0]), np.arctan(tw), atol=1e-6)
+    assert np.isclose(np.arccos(d[0, W // 2] @ [1, 0, 0]), np.arctan(th), atol=1e-6)
+    # one pixel step: the corner grid, not the pixel centres
+    step = unit(c00 + w_vec / W - cam)
+    assert np.allclose(d[0, 1], step, atol=1e-6)

@@ -267,6 +267,16 @@ def main():
         sptr)), max(1, args.warmup // 2), args.steps)
     del d_records
 
+    # the light-probe field's tracing step (genLightProbField.comp with TRACING_STEP): 64 rays per probe
+    # of a 16^3 field over the scene box, each a closest hit plus its sun and leaf-light shadow rays
+    probe_dirs = H.probe_directions(7)
+    n_probe_rays = 16 * 16 * 16 * B.RTQ_PROBE_RAYS
+    d_probe_records = dev.alloc(32 * n_probe_rays)
+    probe_ms = timed(lambda: B.check_device("rtq_trace_probe_field", dev.lib.rtq_trace_probe_field(
+        scene.handle, C.byref(lpf), probe_dirs.ctypes.data, sun.ctypes.data, B.RTQ_SHADOW_SUN, d_probe_records.ptr,
+        sptr)), max(1, args.warmup // 2), args.steps)
+    del d_probe_records
+
     # the same rays in the other buffer order, for comparison (same kernel, same per-ray work)
     d_rays.write(rays_other)
     other_ms = timed(closest, max(1, args.warmup // 2), args.steps)
@@ -325,6 +335,9 @@ def main():
         "record_pass": {"kernel": "kTraceRecord", "ms_per_step": round(record_ms, 4),
                         "mpixels_s": round(n / (record_ms * 1e-3) / 1e6, 2),
                         "per_pixel": "closest hit + sun and leaf-light shadow rays + 8 probe segments (16^3 field)"},
+        "probe_pass": {"kernel": "kTraceProbes", "ms_per_step": round(probe_ms, 4), "rays": n_probe_rays,
+                       "mrays_s": round(n_probe_rays / (probe_ms * 1e-3) / 1e6, 2),
+                       "per_ray": "closest hit + sun and leaf-light shadow rays, 64 rays per probe of a 16^3 field"},
         "other_ray_order": {"ray_order": other_order, "ms_per_step": round(other_ms, 4),
                             "mrays_s": round(n / (other_ms * 1e-3) / 1e6, 2)},
         "host": {"threads": threads, "bvh_build_s": round(build_s, 2), "setup_s": round(setup_s, 2)},

@@ -327,6 +327,36 @@ def test_degenerate_directions_box_scene():
     assert np.array_equal(ga, oa)
 
 
+def test_full_size_soup_bounces_sorted_refill():
+    """Config 3 at full size through the dispatch its benchmark uses (lane refill, rays sorted): the oracle
+    on random samples of both bounces, and on every ray the property any-hit(tmax) == closest hit found
+    (the any-hit walk tests boxes against tmax, so it reaches every leaf the closest-hit walk reaches)."""
+    _print("case: full 3M-tri soup, 2 x 4Mi diffuse bounce rays, sorted lane-refill dispatch, oracle on samples")
+    s = G.scene(3, 1.0)
+    d = G.dev()
+    keep = (d.get_option(B.RTQ_OPTION_PERSISTENT), d.get_option(B.RTQ_OPTION_SORT_RAYS))
+    d.set_option(B.RTQ_OPTION_PERSISTENT, 1)
+    d.set_option(B.RTQ_OPTION_SORT_RAYS, 1)
+    try:
+        b = s["builder"]
+        rng = np.random.default_rng(31)
+        rays = b.surface_rays(seed=3, count=4 << 20)
+        for bounce in (1, 2):
+            g = s["gpu"].trace_closest(rays)
+            idx = rng.choice(rays.size, 16384, replace=False)
+            o = s["orc"].trace_closest(rays[idx], threads=16)
+            G.assert_closest_equal({k: g[k][idx] for k in ("tri_id", "nid", "t", "u", "v")}, o, f"soup bounce {bounce}")
+            found = g["tri_id"] != B.RTQ_NO_HIT
+            assert 0.05 < found.mean() < 0.999
+            assert (g["t"][~found] == rays["tmax"][~found]).all() and (g["t"][found] < rays["tmax"][found]).all()
+            assert np.array_equal(s["gpu"].trace_any(rays).astype(bool), found)
+            if bounce == 1:
+                rays = b.bounce_rays(4, rays, g["hits"])
+    finally:
+        d.set_option(B.RTQ_OPTION_PERSISTENT, keep[0])
+        d.set_option(B.RTQ_OPTION_SORT_RAYS, keep[1])
+
+
 def test_full_size_spheres_subsample():
     _print("case: full 1M-tri sphere scene, 1920x1080 rays, oracle on a 65536-ray sample")
     s = G.scene(2, 1.0)

@@ -119,7 +119,8 @@ rtq_status rtq_device_destroy(rtq_device* dev);
  * the scene box, then direction octant (rocPRIM radix sort) and traces them in that order, so that a
  * wave's lanes hold rays that start close together and head the same way; results still land at
  * each ray's own index and are identical. The sort workspace belongs to the device and grows on
- * demand: sorting dispatches on one device must be issued on one stream. Default 0. */
+ * demand; sorting dispatches on different streams take it in turn (each waits on the device for the
+ * previous one's traversal to finish), so they are safe from any streams but do not overlap. Default 0. */
 #define RTQ_OPTION_SORT_RAYS 3u
 /* RTQ_OPTION_LEAF_WAIT: a wave interleaves descent toward the next leaf with the tests of the leaves
  * its lanes have reached; a round of descent ends once this many of its 64 lanes are ready for leaf

@@ -150,6 +150,39 @@ def test_dispatch_modes_identical(persistent, blocks, sort):
         d.set_option(B.RTQ_OPTION_SORT_RAYS, keep[2])
 
 
+def test_sorted_dispatches_on_two_streams():
+    """Sorting dispatches share the device's sort workspace; issued back to back on two streams with no
+    host synchronisation in between (the second one growing the workspace, on a fresh device), each must
+    still trace its own rays in its own sorted order."""
+    _print("case: sorted dispatches on two streams, fresh device, workspace growth, soup + spheres")
+    d = D.Device(0)
+    d.set_option(B.RTQ_OPTION_PERSISTENT, 1)
+    d.set_option(B.RTQ_OPTION_SORT_RAYS, 1)
+    s3, s2 = G.scene(3, 1 / 48), G.scene(2, 1 / 16)
+    soup = D.Scene(d, s3["blob"], s3["ranges"], s3["pos"])
+    spheres = D.Scene(d, s2["blob"], s2["ranges"], s2["pos"])
+    f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 800 / 600)
+    jobs = [(soup, s3, s3["builder"].surface_rays(seed=21, count=60_000)),
+            (spheres, s2, H.frustum_rays(f, 800, 600, seed=22)),
+            (soup, s3, s3["builder"].surface_rays(seed=23, count=250_000))]
+    streams = [d.stream(), d.stream()]
+    bufs = []
+    for k, (sc, _, rays) in enumerate(jobs):
+        n = rays.size
+        d_r, d_h, d_n = d.upload(rays), d.alloc(16 * n), d.alloc(4 * n)
+        bufs.append((d_r, d_h, d_n))
+    for k, (sc, _, rays) in enumerate(jobs):
+        d_r, d_h, d_n = bufs[k]
+        sc.dispatch(d_r.ptr, rays.size, B.RTQ_TRACE_CLOSEST, d_h.ptr, d_n.ptr, stream=streams[k % 2].ptr)
+    for st in streams:
+        st.synchronize()
+    for k, (sc, ref, rays) in enumerate(jobs):
+        d_r, d_h, d_n = bufs[k]
+        hits = d_h.read(B.HIT_DTYPE, rays.size)
+        g = {"tri_id": hits["tri_id"], "t": hits["t"], "u": hits["u"], "v": hits["v"], "nid": d_n.read(np.uint32, rays.size)}
+        G.assert_closest_equal(g, ref["orc"].trace_closest(rays, threads=16), f"job {k}")
+
+
 @pytest.mark.parametrize("persistent", [0, 1])
 @pytest.mark.parametrize("wait", [1, 7, 40, 64])
 def test_leaf_wait_identical(persistent, wait):

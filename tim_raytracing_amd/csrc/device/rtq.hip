@@ -29,10 +29,12 @@ struct rtq_device {
   uint32_t sortRays = 0;
   uint32_t leafWait = 0;  // 0: rtq::kGridWait for grid dispatches, rtq::kRefillWait for refilling ones
   // ray-sort workspace (keys, values and rocPRIM's temporary storage), grown on demand and shared by
-  // every sorting dispatch on this device
+  // every sorting dispatch on this device; `sortDone` marks the end of the last dispatch that read it
   std::mutex sortMu;
   void* sortBuf = nullptr;
   size_t sortBytes = 0;
+  hipEvent_t sortDone = nullptr;
+  bool sortUsed = false;
 };
 
 // Work counters for persistent dispatches: a small ring so that dispatches in flight on different
@@ -144,10 +146,11 @@ rtq_status rtq_device_create(int hip_ordinal, rtq_device** out) {
 }
 
 rtq_status rtq_device_destroy(rtq_device* dev) {
-  if (dev && dev->sortBuf) {
+  if (dev && (dev->sortBuf || dev->sortDone)) {
     hipSetDevice(dev->ordinal);
     hipDeviceSynchronize();
-    hipFree(dev->sortBuf);
+    if (dev->sortBuf) hipFree(dev->sortBuf);
+    if (dev->sortDone) hipEventDestroy(dev->sortDone);
   }
   delete dev;
   return RTQ_OK;
@@ -369,12 +372,15 @@ rtq_status rtq_scene_get_info(const rtq_scene* scene, rtq_scene_info* info) {
 }
 
 // Sort keys and rocPRIM radix sort on `st`; *perm receives the device permutation (lane order -> ray).
-// The lock keeps the device's workspace for this dispatch until its kernels are enqueued; enqueued work
-// of two streams could still overlap, so sorting dispatches on one device must share a stream.
+// The lock keeps the device's workspace for this dispatch until its kernels are enqueued, and `st`
+// first waits for the event the previous sorting dispatch recorded after its traversal kernel (see
+// sortRelease), so sorting dispatches on different streams take the workspace one after another.
 static rtq_status sortRays(rtq_scene* s, const float4* rays, uint32_t n, hipStream_t st,
                            std::unique_lock<std::mutex>& lock, const uint32_t** perm) {
   rtq_device* dev = s->dev;
   lock = std::unique_lock<std::mutex>(dev->sortMu);
+  if (!dev->sortDone) RTQ_HIP(hipEventCreateWithFlags(&dev->sortDone, hipEventDisableTiming));
+  if (dev->sortUsed) RTQ_HIP(hipStreamWaitEvent(st, dev->sortDone, 0));
   size_t tempBytes = 0;
   RTQ_HIP(rocprim::radix_sort_pairs(nullptr, tempBytes, static_cast<const uint32_t*>(nullptr),
                                     static_cast<uint32_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
@@ -383,7 +389,7 @@ static rtq_status sortRays(rtq_scene* s, const float4* rays, uint32_t n, hipStre
   const size_t need = 4 * arr + tempBytes;
   if (need > dev->sortBytes) {
     if (dev->sortBuf) {
-      RTQ_HIP(hipStreamSynchronize(st));
+      RTQ_HIP(hipEventSynchronize(dev->sortDone));
       RTQ_HIP(hipFree(dev->sortBuf));
       dev->sortBuf = nullptr;
       dev->sortBytes = 0;
@@ -406,7 +412,18 @@ static rtq_status sortRays(rtq_scene* s, const float4* rays, uint32_t n, hipStre
                      s->boxMin[2], sc[0], sc[1], sc[2], keysIn, valsIn);
   RTQ_HIP(hipGetLastError());
   RTQ_HIP(rocprim::radix_sort_pairs(temp, tempBytes, keysIn, keysOut, valsIn, valsOut, n, 0u, 30u, st));
+  // marks the sort's end in case the traversal launch fails; sortRelease moves the mark past it
+  RTQ_HIP(hipEventRecord(dev->sortDone, st));
+  dev->sortUsed = true;
   *perm = valsOut;
+  return RTQ_OK;
+}
+
+// After the traversal kernel that reads the permutation is enqueued on `st`: the next sorting
+// dispatch waits for this point before it rewrites the workspace. Called with the lock held.
+static rtq_status sortRelease(rtq_device* dev, hipStream_t st) {
+  RTQ_HIP(hipEventRecord(dev->sortDone, st));
+  dev->sortUsed = true;
   return RTQ_OK;
 }
 
@@ -441,7 +458,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
                            refillWait);
     });
     RTQ_HIP(hipGetLastError());
-    return RTQ_OK;
+    return perm ? sortRelease(s->dev, st) : RTQ_OK;
   }
   if (!counters && s->dev->persistent) {
     uint32_t* counter = s->counters + rtq::kXcds * (s->nextCounter.fetch_add(1) % kCounterRing);
@@ -458,7 +475,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
                            counter, perm, refillWait);
     });
     RTQ_HIP(hipGetLastError());
-    return RTQ_OK;
+    return perm ? sortRelease(s->dev, st) : RTQ_OK;
   }
   withStack(s->info.max_depth, [&](auto S) {
     constexpr int K = decltype(S)::value;
@@ -479,7 +496,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
     }
   });
   RTQ_HIP(hipGetLastError());
-  return RTQ_OK;
+  return perm ? sortRelease(s->dev, st) : RTQ_OK;
 }
 
 rtq_status rtq_dispatch_rays(rtq_scene* scene, const rtq_ray* d_rays, uint32_t num_rays, uint32_t flags, void* d_out,

@@ -273,12 +273,15 @@ rtq_status rtq_batch_create(rtq_device* dev, rtq_scene* const* scenes, uint32_t 
 rtq_status rtq_batch_destroy(rtq_batch* batch);
 rtq_status rtq_batch_set_params(rtq_batch* batch, const rtq_batch_params* params);
 rtq_status rtq_batch_get_params(const rtq_batch* batch, rtq_batch_params* params);
-/* One persistent launch over every item (asynchronous on `stream`). */
+/* One persistent launch over every item (asynchronous on `stream`). Launches of one batch share its
+ * work counter and item table: issue them on one stream, or order them with events. The scenes must
+ * outlive the batch (it keeps their device views). */
 rtq_status rtq_batch_launch(rtq_batch* batch, void* stream);
 /* The same items as one plain rtq_dispatch_rays launch each (for comparison). */
 rtq_status rtq_batch_launch_plain(rtq_batch* batch, void* stream);
-/* Times every valid (chunk_rays incl. RTQ_BATCH_REFILL, blocks_per_cu, stack_depth) on this batch, `reps` launches each,
- * keeps the fastest in the batch and returns it with its mean time. Synchronises `stream`. */
+/* Times every valid (chunk_rays incl. RTQ_BATCH_REFILL, blocks_per_cu, stack_depth) on this batch, `reps`
+ * launches each, then leaf_wait in 8..64 for the fastest of them; keeps the fastest in the batch and
+ * returns it with its mean time. Synchronises `stream`. */
 rtq_status rtq_batch_autotune(rtq_batch* batch, uint32_t reps, rtq_batch_params* best, float* best_ms,
                               void* stream);
 
@@ -310,7 +313,9 @@ typedef struct rtq_request_stats {
 
 rtq_status rtq_server_create(rtq_device* dev, rtq_scene* const* scenes, uint32_t num_scenes,
                              const rtq_server_config* cfg, rtq_server** out);
-/* Waits for all admitted work, fails queued requests with RTQ_ERR_CANCELLED, frees the server. */
+/* Waits for all admitted work, fails queued requests with RTQ_ERR_CANCELLED, frees the server.
+ * Request handles not yet released stay valid (all finished) and are released as before; no other
+ * call on the server or its requests may run concurrently with this one. */
 rtq_status rtq_server_destroy(rtq_server* srv);
 /* host_rays, host_out (rtq_hit or uint32_t per ray) and host_leaf_nid (closest hit, may be NULL)
  * must stay valid until the request completes. RTQ_ERR_BUDGET: the request alone exceeds the budget. */

@@ -124,3 +124,28 @@ def test_serve_admission_limits():
         h.release()
     assert cancelled >= 1
     srv.close()
+
+
+def test_serve_requests_outlive_server():
+    """Destroying the server with requests queued and in flight: admitted ones complete with correct
+    results, queued ones are cancelled, and every handle can still be waited on, queried and released
+    afterwards."""
+    print("case: server destroyed before its request handles are released", flush=True)
+    dev = G.dev()
+    b = _scenes(1, 11)[0]
+    sc = D.Scene.from_builder(dev, b)
+    srv = D.Server(dev, [sc], budget_bytes=4 << 20, max_inflight=1)
+    rays = [b.surface_rays(seed=40 + k, count=30000) for k in range(12)]
+    hs = [srv.submit(0, B.RTQ_TRACE_ANY, r) for r in rays]
+    srv.close()
+    done = 0
+    for h, r in zip(hs, rays):
+        st = h.wait()
+        assert st in (B.RTQ_OK, 9)  # 9: RTQ_ERR_CANCELLED
+        if st == B.RTQ_OK:
+            assert np.array_equal(h.out, sc.trace_any(r))
+            assert h.stats().latency_ms > 0
+            done += 1
+        assert h.cancel() != B.RTQ_OK  # finished: nothing left to cancel
+        h.release()
+    assert done >= 1

@@ -265,7 +265,7 @@ __device__ __forceinline__ bool traverseAuto(const DevScene& s, uint32_t* stk, c
 }
 
 __device__ __forceinline__ Ray3 loadRay(const float4* rays, uint32_t i, float& tmax) {
-  const float4 a = rays[2 * i], b = rays[2 * i + 1];
+  const float4 a = rays[2 * size_t(i)], b = rays[2 * size_t(i) + 1];
   tmax = a.w;
   return Ray3{a.x, a.y, a.z, b.x, b.y, b.z};
 }
@@ -290,9 +290,9 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceClosest(DevScene s0, const 
   hits[i] = make_float4(__uint_as_float(h.tri), h.t, h.u, h.v);
   if (leafNid) leafNid[i] = h.nid;
   if (COUNT) {
-    counters[3 * i] = c.inner;
-    counters[3 * i + 1] = c.leaves;
-    counters[3 * i + 2] = c.tests;
+    counters[3 * size_t(i)] = c.inner;
+    counters[3 * size_t(i) + 1] = c.leaves;
+    counters[3 * size_t(i) + 2] = c.tests;
   }
 }
 
@@ -312,9 +312,9 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s0, const floa
   const bool occ = traverseAuto<true, COUNT, STACK>(s, lds + threadIdx.x, r, tmax, h, c, wait);
   occluded[i] = occ ? 1u : 0u;
   if (COUNT) {
-    counters[3 * i] = c.inner;
-    counters[3 * i + 1] = c.leaves;
-    counters[3 * i + 2] = c.tests;
+    counters[3 * size_t(i)] = c.inner;
+    counters[3 * size_t(i) + 1] = c.leaves;
+    counters[3 * size_t(i) + 2] = c.tests;
   }
 }
 
@@ -643,9 +643,9 @@ __device__ __forceinline__ void traceRecord(const DevScene& s, uint32_t* stk, co
   r0 = make_uint4(0u, 0u, 0u, __float_as_uint(h.t));
   r1 = make_uint4(0u, 0u, 0u, 0u);
   if (h.tri != 0xFFFFFFFFu) {
-    r0.x = s.triangles[3 * h.tri];
-    r0.y = s.triangles[3 * h.tri + 1];
-    r0.z = s.triangles[3 * h.tri + 2];
+    r0.x = s.triangles[3 * size_t(h.tri)];
+    r0.y = s.triangles[3 * size_t(h.tri) + 1];
+    r0.z = s.triangles[3 * size_t(h.tri) + 2];
   }
   if (h.t < kTMaxF && h.nid != 0xFFFFFFFFu) {
     // hitPos = from + dir * t (baseRaytracingPass.glsl:91)

@@ -408,7 +408,7 @@ static rtq_status sortRays(rtq_scene* s, const float4* rays, uint32_t n, hipStre
     const float ext = s->boxMax[k] - s->boxMin[k];
     sc[k] = ext > 0.0f ? 512.0f / ext : 0.0f;
   }
-  hipLaunchKernelGGL(rtq::kRayKeys, dim3((n + 255) / 256), dim3(256), 0, st, rays, n, s->boxMin[0], s->boxMin[1],
+  hipLaunchKernelGGL(rtq::kRayKeys, dim3(uint32_t((uint64_t(n) + 255) / 256)), dim3(256), 0, st, rays, n, s->boxMin[0], s->boxMin[1],
                      s->boxMin[2], sc[0], sc[1], sc[2], keysIn, valsIn);
   RTQ_HIP(hipGetLastError());
   RTQ_HIP(rocprim::radix_sort_pairs(temp, tempBytes, keysIn, keysOut, valsIn, valsOut, n, 0u, 30u, st));
@@ -435,7 +435,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   if (flags != RTQ_TRACE_CLOSEST && flags != RTQ_TRACE_ANY) return fail(RTQ_ERR_INVALID_ARGUMENT, "bad flags");
   RTQ_HIP(hipSetDevice(s->dev->ordinal));
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
+  const dim3 grid(uint32_t((uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock)), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(rays);
   const uint32_t gridWait = s->dev->leafWait ? s->dev->leafWait : rtq::kGridWait;
   const uint32_t refillWait = s->dev->leafWait ? s->dev->leafWait : rtq::kRefillWait;
@@ -447,7 +447,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   }
   if (!counters && s->dev->persistent == 2) {
     const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
-    const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, n + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
+    const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
     withStack(s->info.max_depth, [&](auto S) {
       constexpr int K = decltype(S)::value;
       if (flags == RTQ_TRACE_CLOSEST)
@@ -464,7 +464,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
     uint32_t* counter = s->counters + rtq::kXcds * (s->nextCounter.fetch_add(1) % kCounterRing);
     hipLaunchKernelGGL(rtq::kResetCounter, dim3(1), dim3(64), 0, st, counter);
     const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
-    const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, n + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
+    const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
     withStack(s->info.max_depth, [&](auto S) {
       constexpr int K = decltype(S)::value;
       if (flags == RTQ_TRACE_CLOSEST)
@@ -607,7 +607,7 @@ rtq_status rtq_shadow_masks(rtq_scene* s, const rtq_ray* d_rays, const rtq_hit* 
   if (sun_dir) std::memcpy(sd, sun_dir, sizeof(sd));
   RTQ_HIP(hipSetDevice(s->dev->ordinal));
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const dim3 grid((n + rtq::kTraceBlock - 1) / rtq::kTraceBlock), block(rtq::kTraceBlock);
+  const dim3 grid(uint32_t((uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock)), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(d_rays);
   const float4* h = reinterpret_cast<const float4*>(d_hits);
   withStack(s->info.max_depth, [&](auto S) {

@@ -68,6 +68,7 @@ void checkScene(const Built& s) {
   std::vector<uint32_t> pairOrder;  // pair bases in depth-first (child 0 first) order
   std::vector<uint32_t> slotDepth(h.nodes.size(), 0xFFFFFFFFu);
   uint32_t inner = 0, leaves = 0;
+  bool ordered = true;  // every reachable inner node's two boxes have min <= max per axis
   // walk device refs and reference links side by side
   struct Item { uint32_t ref, id, depth; };
   std::vector<Item> st;
@@ -116,6 +117,7 @@ void checkScene(const Built& s) {
     const float want[12] = {f[4], f[5], f[12], f[6], f[7], f[13], f[8], f[9], f[14], f[10], f[11], f[15]};
     const float got[12] = {d.a[0], d.a[1], d.a[2], d.a[3], d.b[0], d.b[1], d.b[2], d.b[3], d.c[0], d.c[1], d.c[2], d.c[3]};
     CHECK(std::memcmp(want, got, sizeof(want)) == 0);
+    for (int k = 0; k < 3; ++k) ordered = ordered && want[k] <= want[3 + k] && want[6 + k] <= want[9 + k];
     const uint32_t packed = nodes[16 * it.id + 2];
     const uint32_t l = packed & 0x3FFFFFFFu;
     const bool leafChild[2] = {(packed & 0x80000000u) != 0, (packed & 0x40000000u) != 0};
@@ -134,6 +136,7 @@ void checkScene(const Built& s) {
     for (int k = 1; k >= 0; --k) st.push_back({d.d[k], l + uint32_t(k), it.depth + 1});
   }
   CHECK(inner == h.numInner);
+  CHECK(h.orderedBoxes == ordered);
   CHECK(usedSlots.size() == h.tris.size());
   CHECK(h.nodes.size() == (numNodes == 1 ? 0u : 2 + 2 * pairOrder.size()));
   // the first topSlots records hold the top levels breadth first: depths never decrease along them and
@@ -182,6 +185,29 @@ void checkRejects(Built s) {
                             &why) == RTQ_OK);
 }
 
+// The ordered-box flag (which lets the kernels skip the per-axis min/max of the slab values): set for
+// builder output, cleared by one inverted box or one NaN bound.
+void checkOrderedFlag(Built s) {
+  rtq::HostScene h;
+  const char* why = "";
+  const uint32_t n3 = uint32_t(s.pos.size() / 3);
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), n3, h, &why) == RTQ_OK);
+  CHECK(h.orderedBoxes && h.fastBoxes);
+  float* f = reinterpret_cast<float*>(s.blob.data() + s.ranges.nodes.offset);  // node 0 (the root) is inner
+  std::swap(f[14], f[15]);  // box 1 z: min above max
+  CHECK(f[14] != f[15]);
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), n3, h, &why) == RTQ_OK);
+  CHECK(!h.orderedBoxes && h.fastBoxes);
+  std::swap(f[14], f[15]);
+  const float keep = f[4];
+  f[4] = __builtin_nanf("");  // box 0 min x
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), n3, h, &why) == RTQ_OK);
+  CHECK(!h.orderedBoxes && !h.fastBoxes);
+  f[4] = keep;
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), n3, h, &why) == RTQ_OK);
+  CHECK(h.orderedBoxes && h.fastBoxes);
+}
+
 }  // namespace
 
 int main() {
@@ -193,6 +219,8 @@ int main() {
   const std::vector<float> two = {0, 0, 0, 1, 0, 0, 0, 1, 0, 5, 5, 5, 6, 5, 5, 5, 6, 5};
   checkScene(build(0, 0, 0, &two));
   checkRejects(build(RTQ_SCENE_SPHERES, 4, 8));
+  checkOrderedFlag(build(RTQ_SCENE_SPHERES, 5, 16));
+  checkOrderedFlag(build(RTQ_SCENE_SOUP, 6, 2000));
   if (g_fail) return 1;
   std::puts("ok");
   return 0;

@@ -87,10 +87,12 @@ def test_batch_matches_plain_and_oracle():
     for chunk, blocks, stack, graph, wait in ((64, 2, 24, 0, 0), (128, 4, 32, 0, 5), (512, 8, 64, 0, 64),
                                               (256, 6, 32, 1, 0), (1, 8, 24, 0, 0), (1, 2, 64, 0, 1),
                                               (1, 12, 32, 1, 48), (1, 8, 24, 0, 64)):
-        for (d_r, d_o, d_n) in bufs:
-            d_o.write(np.zeros(d_o.nbytes, np.uint8))
         batch.set_params(chunk_rays=chunk, blocks_per_cu=blocks, stack_depth=stack, use_graph=graph, leaf_wait=wait)
-        for _ in range(3 if graph else 1):   # graph replays must all give the same result
+        for _ in range(3 if graph else 1):   # every graph replay must redo the work: outputs cleared before each
+            for (d_r, d_o, d_n) in bufs:
+                d_o.write(np.zeros(d_o.nbytes, np.uint8))
+                if d_n is not None:
+                    d_n.write(np.zeros(d_n.nbytes, np.uint8))
             batch.launch(sptr)
             stream.synchronize()
             got = read_all()

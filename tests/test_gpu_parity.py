@@ -334,6 +334,28 @@ def test_edge_cases():
         gs.dispatch(buf.ptr, 1, 7, buf.ptr, None)
 
 
+def test_swapped_box_planes_keep_general_form():
+    """Swapping the min and max planes of one axis of a box changes nothing under the reference's
+    per-axis min/max, but the octant form of the box test (used when a wave's rays share an octant)
+    takes min <= max for granted: the upload must notice and keep such a scene on the general form."""
+    _print("case: root child-1 box with swapped z planes, spheres 1/16, camera + row-major rays, oracle")
+    s = G.scene(2, 1 / 16)
+    blob = s["blob"].copy()
+    root = blob[s["ranges"].nodes.offset:s["ranges"].nodes.offset + 64].view(np.float32)
+    root[[14, 15]] = root[[15, 14]]
+    assert root[14] > root[15]
+    swapped = {"gpu": D.Scene(G.dev(), blob, s["ranges"], s["pos"]), "orc": oracle.Scene(blob, s["ranges"], s["pos"])}
+    f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 640 / 480)
+    rays = H.frustum_rays(f, 640, 480, seed=12)
+    g = swapped["gpu"].trace_closest(rays)
+    G.assert_closest_equal(g, swapped["orc"].trace_closest(rays, threads=16), "swapped planes")
+    G.assert_closest_equal(g, s["gpu"].trace_closest(rays), "swapped vs original")
+    cam = swapped["gpu"].trace_camera(f, 640, 480)
+    G.assert_closest_equal(cam, s["gpu"].trace_camera(f, 640, 480), "camera pass, swapped vs original")
+    rays["tmax"] = np.float32(30.0)
+    assert np.array_equal(swapped["gpu"].trace_any(rays), swapped["orc"].trace_any(rays, threads=16)[0])
+
+
 def test_degenerate_directions_box_scene():
     _print("case: closed box, axis-parallel and slab-plane rays")
     s = G.scene(1)

@@ -88,6 +88,23 @@ __device__ __forceinline__ float boxHitFast(const Ray3& r, const RayRcp& q, floa
   return (tmin < tm) ? tmin : -1.0f;
 }
 
+// boxHitFast for a ray whose direction octant is OCT (bit k set: component k negative) and a box with
+// min <= max on every axis. quotient() is monotone in x (x -> RN32(RN64(x * r)) with one fixed r), so
+// on an axis where d > 0 the min-plane value is the smaller of the two and on one where d < 0 the
+// larger: the per-axis min and max of boxHitFast are known without comparing, and the entry and exit
+// distances are the same values (up to the sign of a zero, which the comparisons ignore).
+template <int OCT>
+__device__ __forceinline__ float boxHitOrd(const Ray3& r, const RayRcp& q, float mnx, float mny, float mnz,
+                                           float mxx, float mxy, float mxz, float tmax) {
+  constexpr bool nx = (OCT & 1) != 0, ny = (OCT & 2) != 0, nz = (OCT & 4) != 0;
+  const float enx = quotient((nx ? mxx : mnx) - r.ox, q.rx), fax = quotient((nx ? mnx : mxx) - r.ox, q.rx);
+  const float eny = quotient((ny ? mxy : mny) - r.oy, q.ry), fay = quotient((ny ? mny : mxy) - r.oy, q.ry);
+  const float enz = quotient((nz ? mxz : mnz) - r.oz, q.rz), faz = quotient((nz ? mnz : mxz) - r.oz, q.rz);
+  const float tmin = fmaxf(0.0f, fmaxf(enx, fmaxf(eny, enz)));
+  const float tm = fminf(tmax, fminf(fax, fminf(fay, faz)));
+  return (tmin < tm) ? tmin : -1.0f;
+}
+
 // CollideTriangle (collision.glsl:58-95) with edges precomputed. Every quantity is computed with the
 // shader's operations in its order, and the early returns are folded into one final condition, so the
 // wave runs one straight-line block (its loads issue together) instead of three divergent branches; an
@@ -165,7 +182,9 @@ __device__ __forceinline__ DevScene stageTop(const DevScene& s) {
 
 // One round of descent for the calling lanes. FASTONLY: every calling lane takes the reciprocal slab
 // form (decided per wave), so no per-lane branch is needed; otherwise `fast` chooses per lane.
-template <int STACK, bool FASTONLY, bool COUNT>
+// OCT >= 0 (with FASTONLY): every lane's direction lies in octant OCT and the scene's boxes are
+// ordered, so boxHitOrd replaces boxHitFast.
+template <int STACK, bool FASTONLY, bool COUNT, int OCT = -1>
 __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, const DevNode* top, uint32_t topSlots,
                                              uint32_t* stk, const Ray3& r, const RayRcp& rq, bool fast, float lim,
                                              uint32_t& node, int& sp, uint32_t wait, uint32_t extra,
@@ -186,7 +205,10 @@ __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, 
       const uint4 d = reinterpret_cast<const uint4*>(np)[3];
       if (COUNT) ++inner;
       float d0, d1;
-      if (FASTONLY || fast) {
+      if constexpr (OCT >= 0) {
+        d0 = boxHitOrd<OCT>(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+        d1 = boxHitOrd<OCT>(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+      } else if (FASTONLY || fast) {
         d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
         d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
       } else {
@@ -210,7 +232,7 @@ __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, 
 
 // Walk the tree for one ray. ANY: stop at the first accepted triangle (returns true).
 // `stk` points at this lane's column of the block's LDS stack (stride kTraceBlock).
-template <bool ANY, bool COUNT, int STACK, bool FAST>
+template <bool ANY, bool COUNT, int STACK, bool FAST, int OCT = -1>
 __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const Ray3& r, float tmax, Closest& hit,
                                          Counters& cnt, uint32_t wait) {
   RayRcp rq{};
@@ -219,8 +241,8 @@ __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const
   int sp = 0;
   if (!ANY) hit.t = tmax;
   for (;;) {
-    descendRound<STACK, FAST, COUNT>(s.nodes, s.top, s.topSlots, stk, r, rq, FAST, ANY ? tmax : hit.t, node, sp, wait,
-                                     0u, cnt.inner);
+    descendRound<STACK, FAST, COUNT, OCT>(s.nodes, s.top, s.topSlots, stk, r, rq, FAST, ANY ? tmax : hit.t, node, sp,
+                                          wait, 0u, cnt.inner);
     if (node == kRefDone) break;
     if (node < kRefDone) continue;  // still descending: next round
     if (COUNT) cnt.leaves++;
@@ -256,12 +278,28 @@ __device__ __forceinline__ bool fastLane(const DevScene& s, const Ray3& r, float
          fastCoord(r.ox) && fastCoord(r.oy) && fastCoord(r.oz) && tmax == tmax;
 }
 
-// One traversal routine per wave: the fast slab division when every active lane qualifies.
+// One traversal routine per wave: the fast slab division when every active lane qualifies, and its
+// octant form when, in addition, the scene's boxes are ordered and all the wave's rays head into one
+// octant (camera tiles, sun rays).
 template <bool ANY, bool COUNT, int STACK>
 __device__ __forceinline__ bool traverseAuto(const DevScene& s, uint32_t* stk, const Ray3& r, float tmax,
                                              Closest& hit, Counters& cnt, uint32_t wait = kGridWait) {
-  if (__all(fastLane(s, r, tmax) ? 1 : 0)) return traverse<ANY, COUNT, STACK, true>(s, stk, r, tmax, hit, cnt, wait);
-  return traverse<ANY, COUNT, STACK, false>(s, stk, r, tmax, hit, cnt, wait);
+  if (!__all(fastLane(s, r, tmax) ? 1 : 0)) return traverse<ANY, COUNT, STACK, false>(s, stk, r, tmax, hit, cnt, wait);
+  const uint32_t oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+  const uint32_t oct0 = __builtin_amdgcn_readfirstlane(oct);
+  if (s.orderedBoxes && __all(oct == oct0 ? 1 : 0)) {
+    switch (oct0) {
+      case 0: return traverse<ANY, COUNT, STACK, true, 0>(s, stk, r, tmax, hit, cnt, wait);
+      case 1: return traverse<ANY, COUNT, STACK, true, 1>(s, stk, r, tmax, hit, cnt, wait);
+      case 2: return traverse<ANY, COUNT, STACK, true, 2>(s, stk, r, tmax, hit, cnt, wait);
+      case 3: return traverse<ANY, COUNT, STACK, true, 3>(s, stk, r, tmax, hit, cnt, wait);
+      case 4: return traverse<ANY, COUNT, STACK, true, 4>(s, stk, r, tmax, hit, cnt, wait);
+      case 5: return traverse<ANY, COUNT, STACK, true, 5>(s, stk, r, tmax, hit, cnt, wait);
+      case 6: return traverse<ANY, COUNT, STACK, true, 6>(s, stk, r, tmax, hit, cnt, wait);
+      default: return traverse<ANY, COUNT, STACK, true, 7>(s, stk, r, tmax, hit, cnt, wait);
+    }
+  }
+  return traverse<ANY, COUNT, STACK, true>(s, stk, r, tmax, hit, cnt, wait);
 }
 
 __device__ __forceinline__ Ray3 loadRay(const float4* rays, uint32_t i, float& tmax) {

@@ -56,6 +56,7 @@ struct DevScene {
   uint32_t leafWords;
   uint32_t numLights;
   uint32_t fastBoxes;        // 1 if every inner-node box bound qualifies for the fast slab division
+  uint32_t orderedBoxes;     // 1 if every box has min <= max on each axis (no empty or NaN bounds)
   const uint32_t* triangles; // the blob's Triangle records (3 words each), indexed by triangle id
   uint32_t topSlots;         // leading node records a kernel may read from its LDS copy (<= kTopSlots)
   const DevNode* top;        // that LDS copy, set by the kernel; nullptr outside a staging kernel

@@ -348,6 +348,7 @@ rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, 
   s->view.leafWords = h.leafWords;
   s->view.numLights = h.numLights;
   s->view.fastBoxes = h.fastBoxes ? 1u : 0u;
+  s->view.orderedBoxes = h.orderedBoxes ? 1u : 0u;
   s->view.topSlots = h.topSlots;
   s->view.top = nullptr;
   s->info.num_nodes = h.numNodes;

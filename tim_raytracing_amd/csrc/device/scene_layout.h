@@ -26,6 +26,7 @@ struct HostScene {
   uint32_t topSlots = 0;             // leading node records holding the top levels, breadth first
   uint32_t numNodes = 0, numInner = 0, leafWords = 0, numLights = 0, numTris = 0, maxDepth = 0;
   bool fastBoxes = true;             // every box bound qualifies for the reciprocal slab form
+  bool orderedBoxes = true;          // every box has min <= max on each axis
 };
 
 namespace detail {
@@ -204,6 +205,11 @@ inline rtq_status buildHostScene(const uint8_t* blob, size_t bytes, const rtq_ra
         const float ax = std::fabs(x);
         if (!(ax == 0.0f || (ax >= 0x1p-37f && ax <= 0x1p59f))) h.fastBoxes = false;
       }
+  // box 0 = min (a0, a1, a2), max (a3, b0, b1); box 1 = min (b2, b3, c0), max (c1, c2, c3)
+  for (const DevNode& d : h.nodes)
+    if (!(d.a[0] <= d.a[3] && d.a[1] <= d.b[0] && d.a[2] <= d.b[1] && d.b[2] <= d.c[1] && d.b[3] <= d.c[2] &&
+          d.c[0] <= d.c[3]))
+      h.orderedBoxes = false;
   h.leafOff.resize(numNodes);
   for (uint32_t id = 0; id < numNodes; ++id) h.leafOff[id] = nodes[16 * id + 3];
   h.leafData.assign(leaf, leaf + leafWords);

@@ -159,7 +159,7 @@ def cpu_baseline(blob, ranges, pos, rays, threads: int, target_s: float):
         dt = time.perf_counter() - t0
         if dt >= target_s or passes >= 1000:
             break
-    return {
+    out = {
         "value": round(traced / dt / 1e6, 4),
         "unit": "Mrays/s",
         "cores": threads,
@@ -167,6 +167,16 @@ def cpu_baseline(blob, ranges, pos, rays, threads: int, target_s: float):
         "sample": f"{sample.size} of the {rays.size} primary rays (stride {stride}) x {passes} passes, closest "
                   f"hit, bitstack restatement of traverseBvh, {threads} threads, {dt:.1f} s",
     }
+    if threads > 1:
+        # the same restatement on one thread, over a strided sample sized for about a fifth of target_s
+        one_n = int(min(rays.size, max(1000, traced / dt / threads * target_s / 5)))
+        one = np.ascontiguousarray(rays[:: max(1, rays.size // one_n)][:one_n])
+        t0 = time.perf_counter()
+        sc.trace_closest(one, threads=1)
+        dt1 = time.perf_counter() - t0
+        out["single_thread_value"] = round(one.size / dt1 / 1e6, 4)
+        out["single_thread_sample"] = f"{one.size} primary rays, 1 thread, {dt1:.1f} s"
+    return out
 
 
 def main():

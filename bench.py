@@ -332,6 +332,8 @@ def main():
             "mean_leaves": round(sum_leaf / n, 3),
             "mean_triangle_tests": round(sum_tri / n, 2),
             "roofline_mrays_s": round(HBM_PEAK_GBS * 1e9 / (alg_bytes / n) / 1e6, 1),
+            # reported, not a roof: about 50 flops per inner node (two slab tests) and 45 per triangle test
+            "flops_per_ray": round((50.0 * sum_inner + 45.0 * sum_tri) / n, 1),
         },
         "hit_rate": round(n_hit / n, 4),
         "shadow": {

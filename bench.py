@@ -83,7 +83,13 @@ def tile_order(width: int, height: int, t: int = 8) -> np.ndarray:
 
 
 def stack_class(depth: int) -> int:
+    """Stack depth parameter of the batched and served forms (24, 32 or 64)."""
     return 24 if depth <= 24 else (32 if depth <= 32 else 64)
+
+
+def kernel_stack(depth: int) -> int:
+    """LDS stack entries the single-scene kernels are instantiated with for a tree of this depth."""
+    return 22 if depth <= 22 else stack_class(depth)
 
 
 def configure_dispatch(dev, args, auto: str, auto_sort: int = 0):
@@ -107,8 +113,8 @@ def configure_dispatch(dev, args, auto: str, auto_sort: int = 0):
 def kernel_name(args, depth: int, any_hit: bool = False) -> str:
     if args.dispatch in ("persistent", "range"):
         base = "kTracePersistent" if args.dispatch == "persistent" else "kTraceWaveRange"
-        return f"{base}<{stack_class(depth)},{'true' if any_hit else 'false'}>"
-    return f"{'kTraceAny' if any_hit else 'kTraceClosest'}<{stack_class(depth)},false>"
+        return f"{base}<{kernel_stack(depth)},{'true' if any_hit else 'false'}>"
+    return f"{'kTraceAny' if any_hit else 'kTraceClosest'}<{kernel_stack(depth)},false>"
 
 
 def measured_traffic(workload: str, kernel: str):

@@ -90,13 +90,16 @@ void freeScene(rtq_scene* s) {
   delete s;
 }
 
-// Smallest per-lane LDS stack (entries) that holds the deepest path of a scene: a smaller stack
-// leaves room for more resident waves per CU.
-int stackClass(uint32_t depth) { return depth <= 24 ? 24 : (depth <= 32 ? 32 : 64); }
+// Smallest per-lane LDS stack (entries) that holds the deepest path of a scene (a walk pushes at most
+// one entry per inner node on its path, so a tree whose leaves lie at depth <= d needs d entries): a
+// smaller stack leaves room for more resident waves per CU. With 22 entries a 128-lane block takes
+// 11 KB of LDS, and the grid kernels reach their register limit of 7 waves per SIMD; with 24, 6.5.
+int stackClass(uint32_t depth) { return depth <= 22 ? 22 : (depth <= 24 ? 24 : (depth <= 32 ? 32 : 64)); }
 
 template <class F>
 void withStack(uint32_t depth, F&& f) {
   switch (stackClass(depth)) {
+    case 22: f(std::integral_constant<int, 22>{}); break;
     case 24: f(std::integral_constant<int, 24>{}); break;
     case 32: f(std::integral_constant<int, 32>{}); break;
     default: f(std::integral_constant<int, 64>{}); break;

@@ -412,6 +412,31 @@ def test_full_size_soup_bounces_sorted_refill():
         d.set_option(B.RTQ_OPTION_SORT_RAYS, keep[1])
 
 
+def test_large_dispatch_16m_rays():
+    """One dispatch of 2^24 rays (512 MiB of rays): the grid and the lane-refilling kernel give identical
+    records on every ray, and the oracle agrees on a random sample."""
+    _print("case: 4096x4096 rays in one dispatch, spheres 1/16, grid vs lane refill, oracle sample")
+    s = G.scene(2, 1 / 16)
+    f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 1.0)
+    rays = H.frustum_rays(f, 4096, 4096, seed=41)
+    assert rays.size == 1 << 24
+    d = G.dev()
+    keep = d.get_option(B.RTQ_OPTION_PERSISTENT)
+    try:
+        d.set_option(B.RTQ_OPTION_PERSISTENT, 0)
+        grid = s["gpu"].trace_closest(rays)
+        d.set_option(B.RTQ_OPTION_PERSISTENT, 1)
+        refill = s["gpu"].trace_closest(rays)
+    finally:
+        d.set_option(B.RTQ_OPTION_PERSISTENT, keep)
+    G.assert_closest_equal(grid, refill, "grid vs refill")
+    idx = np.random.default_rng(41).choice(rays.size, 16384, replace=False)
+    idx[:2] = [0, rays.size - 1]  # first and last ray
+    o = s["orc"].trace_closest(rays[idx], threads=16)
+    G.assert_closest_equal({k: grid[k][idx] for k in ("tri_id", "nid", "t", "u", "v")}, o, "oracle sample")
+    assert (grid["tri_id"] != B.RTQ_NO_HIT).mean() > 0.1
+
+
 def test_full_size_spheres_subsample():
     _print("case: full 1M-tri sphere scene, 1920x1080 rays, oracle on a 65536-ray sample")
     s = G.scene(2, 1.0)

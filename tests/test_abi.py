@@ -32,9 +32,16 @@ def test_device_library_exports_all():
     lib = C.CDLL(B.DEVICE_LIB)
     missing = [f for f in declared("rtq.h") if not hasattr(lib, f)]
     assert not missing
-    # status strings need no device
+    # status strings need no device: every code of the header's enum has its own name
     lib.rtq_status_string.restype = C.c_char_p
-    assert lib.rtq_status_string(4) == b"RTQ_ERR_LIMIT"
+    text = open(os.path.join(ROOT, "include", "rtq.h")).read()
+    enum = text[text.index("typedef enum"):text.index("} rtq_status;")]
+    codes = {int(v): k for k, v in re.findall(r"\b(RTQ_(?:OK|ERR_\w+))\s*=\s*(\d+)", enum)}
+    assert len(codes) >= 13 and codes[0] == "RTQ_OK"
+    for value, name in codes.items():
+        assert lib.rtq_status_string(value) == name.encode(), name
+        assert B.STATUS_NAMES[value] == name
+    assert lib.rtq_status_string(999) == b"RTQ_ERR_UNKNOWN"
 
 
 def test_bindings_cover_headers():

@@ -85,8 +85,8 @@ rtq_status uploadArray(rtq_scene* s, const std::vector<T>& host, const T** dptr)
 
 void freeScene(rtq_scene* s) {
   if (!s) return;
-  if (s->dev) hipSetDevice(s->dev->ordinal);
-  for (void* p : s->allocations) hipFree(p);
+  if (s->dev) (void)hipSetDevice(s->dev->ordinal);
+  for (void* p : s->allocations) (void)hipFree(p);
   delete s;
 }
 
@@ -124,6 +124,7 @@ const char* rtq_status_string(rtq_status s) {
     case RTQ_ERR_CANCELLED: return "RTQ_ERR_CANCELLED";
     case RTQ_ERR_TIMEOUT: return "RTQ_ERR_TIMEOUT";
     case RTQ_ERR_INTERNAL: return "RTQ_ERR_INTERNAL";
+    case RTQ_ERR_CACHE_MISS: return "RTQ_ERR_CACHE_MISS";
   }
   return "RTQ_ERR_UNKNOWN";
 }
@@ -150,10 +151,10 @@ rtq_status rtq_device_create(int hip_ordinal, rtq_device** out) {
 
 rtq_status rtq_device_destroy(rtq_device* dev) {
   if (dev && (dev->sortBuf || dev->sortDone)) {
-    hipSetDevice(dev->ordinal);
-    hipDeviceSynchronize();
-    if (dev->sortBuf) hipFree(dev->sortBuf);
-    if (dev->sortDone) hipEventDestroy(dev->sortDone);
+    (void)hipSetDevice(dev->ordinal);
+    (void)hipDeviceSynchronize();
+    if (dev->sortBuf) (void)hipFree(dev->sortBuf);
+    if (dev->sortDone) (void)hipEventDestroy(dev->sortDone);
   }
   delete dev;
   return RTQ_OK;
@@ -638,7 +639,7 @@ rtq_status rtq_selftest_division(rtq_device* dev, uint64_t seed, uint64_t count,
   }
   unsigned long long h = 0;
   if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return hipFail(e, "division self-test");
   *mismatches = h;
   return RTQ_OK;
@@ -667,8 +668,8 @@ rtq_status rtq_cull_lights(rtq_scene* s, const rtq_frustum* f, uint32_t width, u
                      ngy, d_tile_offsets, d_tile_lights, capacity);
   hipError_t le = hipGetLastError();
   uint32_t total = 0;
-  hipMemcpyAsync(&total, d_tile_offsets + ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
-  hipFreeAsync(counts, st);
+  (void)hipMemcpyAsync(&total, d_tile_offsets + ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+  (void)hipFreeAsync(counts, st);
   RTQ_HIP(hipStreamSynchronize(st));
   if (le != hipSuccess) return hipFail(le, "culling kernels");
   if (h_total) *h_total = total;

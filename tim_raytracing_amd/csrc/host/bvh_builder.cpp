@@ -309,11 +309,10 @@ void BvhBuilder::countSplit(const Ctx& ctx, Split& out, const Box3& parent, cons
   }
 }
 
-// computeSplitScore (BVHBuilder.cpp:503-507): min(uniqueLeft, uniqueRight) as float.
-static float splitScore(u32 uL, u32 uR) {
-  u32 total = uL + uR;
-  return float(total - (uL < uR ? uR : uL));
-}
+// computeSplitScore (BVHBuilder.cpp:503-507) is min(uniqueLeft, uniqueRight) converted to float; the
+// searches below compare that minimum as an integer, which orders splits identically while the counts
+// stay below 2^24 (every integer up to 2^24 is a float), so nodes of more than 16 M items could break a
+// tie the reference's float comparison would see differently.
 
 // searchBestSplit (BVHBuilder.cpp:1096-1183).
 bool BvhBuilder::searchSplit(const Ctx& ctx, const TreeNode* node, const std::vector<u32>& items, int axis,

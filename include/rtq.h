@@ -127,6 +127,8 @@ rtq_status rtq_device_destroy(rtq_device* dev);
  * tests (1..64; 64 = every lane reaches its leaf first). 0 (default) = 24 for one ray per lane, 32 for
  * refilling dispatches. Results are identical for every value. */
 #define RTQ_OPTION_LEAF_WAIT 4u
+/* Options apply to dispatches issued after the call; set them while no other thread dispatches on the
+ * device (they are plain fields, read when a dispatch is enqueued). */
 rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t value);
 rtq_status rtq_device_get_option(const rtq_device* dev, uint32_t option, uint32_t* value);
 

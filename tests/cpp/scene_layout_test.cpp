@@ -9,6 +9,7 @@
 
 #include "../../include/rtq_build.h"
 #include "../../tim_raytracing_amd/csrc/device/scene_layout.h"
+#include "../../tim_raytracing_amd/csrc/device/stack_class.h"
 
 namespace {
 
@@ -165,6 +166,48 @@ void checkScene(const Built& s) {
     if (!innerSlots.count(uint32_t(k))) CHECK(std::memcmp(&h.nodes[k], &zero, sizeof(zero)) == 0);
 }
 
+// The kernels' stack discipline in its worst case, where both child boxes of every inner node pass:
+// the far child is pushed, the near one entered, and a finished leaf pops. The deepest stack such a
+// walk reaches never exceeds the tree depth, so the stack class chosen for that depth always holds it.
+void checkStackBound(const Built& s) {
+  rtq::HostScene h;
+  const char* why = "";
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), uint32_t(s.pos.size() / 3), h,
+                            &why) == RTQ_OK);
+  std::vector<uint32_t> stk;
+  size_t deepest = 0;
+  uint32_t node = h.root;
+  for (;;) {
+    while (node < rtq::kRefDone) {
+      const rtq::DevNode& d = h.nodes[node];
+      stk.push_back(d.d[1]);  // far child deferred
+      deepest = std::max(deepest, stk.size());
+      node = d.d[0];
+    }
+    if (stk.empty()) break;
+    node = stk.back();
+    stk.pop_back();
+  }
+  CHECK(deepest <= h.maxDepth);
+  CHECK(rtq::gridStackClass(h.maxDepth) != 0 && rtq::stackHolds(rtq::gridStackClass(h.maxDepth), uint32_t(deepest)));
+  CHECK(rtq::batchStackClass(h.maxDepth) != 0 && rtq::stackHolds(rtq::batchStackClass(h.maxDepth), uint32_t(deepest)));
+}
+
+// The depth -> stack mapping: every depth the upload accepts gets a class that holds it, the classes
+// are the ones the kernels are built for, and deeper trees get none.
+void checkStackClasses() {
+  for (uint32_t d = 0; d <= rtq::kMaxTreeDepth; ++d) {
+    const uint32_t g = rtq::gridStackClass(d), b = rtq::batchStackClass(d);
+    CHECK(g == 22 || g == 24 || g == 32 || g == 64);
+    CHECK(b == 24 || b == 32 || b == 64);
+    CHECK(rtq::stackHolds(g, d) && rtq::stackHolds(b, d));
+    if (d > 0) CHECK(g >= rtq::gridStackClass(d - 1) && b >= rtq::batchStackClass(d - 1));
+  }
+  CHECK(rtq::gridStackClass(21) == 22 && rtq::gridStackClass(23) == 24 && rtq::gridStackClass(25) == 32);
+  CHECK(rtq::gridStackClass(rtq::kMaxTreeDepth + 1) == 0 && rtq::batchStackClass(rtq::kMaxTreeDepth + 1) == 0);
+  CHECK(!rtq::stackHolds(24, 25));
+}
+
 void checkRejects(Built s) {
   rtq::HostScene h;
   const char* why = "";
@@ -218,6 +261,10 @@ int main() {
   checkScene(build(0, 0, 0, &one));  // a single leaf as the root
   const std::vector<float> two = {0, 0, 0, 1, 0, 0, 0, 1, 0, 5, 5, 5, 6, 5, 5, 5, 6, 5};
   checkScene(build(0, 0, 0, &two));
+  checkStackClasses();
+  checkStackBound(build(RTQ_SCENE_BOX, 1, 0));
+  checkStackBound(build(RTQ_SCENE_SPHERES, 2, 64));
+  checkStackBound(build(RTQ_SCENE_SOUP, 3, 3000));
   checkRejects(build(RTQ_SCENE_SPHERES, 4, 8));
   checkOrderedFlag(build(RTQ_SCENE_SPHERES, 5, 16));
   checkOrderedFlag(build(RTQ_SCENE_SOUP, 6, 2000));

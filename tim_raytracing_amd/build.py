@@ -20,7 +20,7 @@ HOST_SRC = [os.path.join(PKG, "csrc", "host", f) for f in ("bvh_builder.cpp", "s
                                                          "capi_build.cpp")]
 DEVICE_SRC = [os.path.join(PKG, "csrc", "device", "rtq.hip")]
 DEVICE_DEPS = [os.path.join(PKG, "csrc", "device", f)
-               for f in ("kernels.h", "layout.h", "scene_layout.h", "batch.inc", "serve.inc", "serve_core.h")]
+               for f in ("kernels.h", "layout.h", "scene_layout.h", "stack_class.h", "batch.inc", "serve.inc", "serve_core.h")]
 
 HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall",
               "-Wextra", "-pthread"]

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "layout.h"
+#include "stack_class.h"
 
 namespace rtq {
 
@@ -218,9 +219,16 @@ __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, 
       const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
       if (h0 && h1) {
         const bool n1 = d1 < d0;
-        if (sp < STACK) stk[sp * kTraceBlock] = n1 ? d.x : d.y;
-        ++sp;
-        node = n1 ? d.y : d.x;
+        if (sp < STACK) {
+          stk[sp * kTraceBlock] = n1 ? d.x : d.y;
+          ++sp;
+          node = n1 ? d.y : d.x;
+        } else {
+          // a deeper tree than the stack holds: the lane stops here instead of writing past its
+          // stack column (unreachable, every launch sizes STACK to the tree depth, stack_class.h)
+          node = kRefDone;
+          sp = 0;
+        }
       } else if (h0 || h1) {
         node = h0 ? d.x : d.y;
       } else {
@@ -1052,40 +1060,54 @@ __device__ __forceinline__ void tilePlanes(const Frustum4& f, uint32_t ngx, uint
   computePlane(f.cam, bl, br, pl + 12);
 }
 
-// sphereFrustum4Collision (collision.glsl:219-227).
+// sphereFrustum4Collision (collision.glsl:219-227): culled if dot(n, c) + w < -r on any plane. All four
+// tests are evaluated (no early exit, so the wave runs one straight block); the result is the same.
 __device__ __forceinline__ bool sphereInFrustum(const float pl[16], float4 L) {
+  bool in = true;
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    if ((pl[4 * p] * L.x + pl[4 * p + 1] * L.y) + pl[4 * p + 2] * L.z + pl[4 * p + 3] < -L.w) return false;
-  }
-  return true;
+  for (int p = 0; p < 4; ++p) in &= !((pl[4 * p] * L.x + pl[4 * p + 1] * L.y) + pl[4 * p + 2] * L.z + pl[4 * p + 3] < -L.w);
+  return in;
 }
 
 constexpr int kCullBlock = 256;
 
-// Pass 1: lights per tile.
-__global__ __launch_bounds__(kCullBlock) void kCullCount(Frustum4 f, const float4* __restrict__ lights,
-                                                         uint32_t numLights, uint32_t ngx, uint32_t ngy,
-                                                         uint32_t* __restrict__ counts) {
-  const uint32_t tile = blockIdx.x;
-  const uint32_t tx = tile % ngx, ty = tile / ngx;
-  float pl[16];
-  tilePlanes(f, ngx, ngy, tx, ty, pl);
-  uint32_t mine = 0;
-  for (uint32_t i = threadIdx.x; i < numLights; i += kCullBlock) mine += sphereInFrustum(pl, lights[i]) ? 1u : 0u;
-  __shared__ uint32_t red[kCullBlock / 64];
-  // wave reduction then block reduction
-  for (int o = 32; o > 0; o >>= 1) mine += __shfl_down(mine, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mine;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t s = 0;
-    for (int w = 0; w < kCullBlock / 64; ++w) s += red[w];
-    counts[tile] = s;
-  }
+// Scratch of one culling call: a 64-bit light mask per (tile, group of 64 lights), tile-major, then
+// the per-tile light counts. Every word is written by kCullMask before kScan and kCullEmit read it,
+// so the scratch needs no clearing between calls.
+__host__ __device__ constexpr uint64_t cullScratchWords(uint32_t ntiles, uint32_t groups) {
+  return 2ull * ntiles * groups + ntiles;
 }
 
-// Exclusive scan of counts[0..n) into offsets[0..n], one block.
+// Pass 1: one wave per tile tests every light once against the tile's four planes. Lane l of the
+// wave keeps the ballot of group g where g % 64 == l, and the 64 words of a round of groups are
+// stored together (one coalesced store); lane 0 writes the tile's light count.
+__global__ __launch_bounds__(kCullBlock) void kCullMask(Frustum4 f, const float4* __restrict__ lights,
+                                                        uint32_t numLights, uint32_t ngx, uint32_t ngy,
+                                                        uint32_t groups, uint64_t* __restrict__ masks,
+                                                        uint32_t* __restrict__ counts) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t tile = blockIdx.x * (kCullBlock / 64u) + (threadIdx.x >> 6);
+  if (tile >= ngx * ngy) return;
+  float pl[16];
+  tilePlanes(f, ngx, ngy, tile % ngx, tile / ngx, pl);
+  uint32_t count = 0;
+  uint64_t* row = masks + size_t(tile) * groups;
+  for (uint32_t g0 = 0; g0 < groups; g0 += 64u) {
+    uint64_t mine = 0;
+    const uint32_t gEnd = min(groups, g0 + 64u);
+    for (uint32_t g = g0; g < gEnd; ++g) {
+      const uint32_t i = g * 64u + lane;
+      const bool in = i < numLights && sphereInFrustum(pl, lights[i]);
+      const uint64_t m = __ballot(in);
+      count += uint32_t(__popcll(m));
+      if (lane == g - g0) mine = m;
+    }
+    if (g0 + lane < gEnd) row[g0 + lane] = mine;
+  }
+  if (lane == 0) counts[tile] = count;
+}
+
+// Pass 2: exclusive scan of counts[0..n) into offsets[0..n], one block; offsets[n] is the total.
 __global__ __launch_bounds__(1024) void kScan(const uint32_t* __restrict__ counts, uint32_t n,
                                               uint32_t* __restrict__ offsets) {
   __shared__ uint32_t part[1024];
@@ -1109,35 +1131,34 @@ __global__ __launch_bounds__(1024) void kScan(const uint32_t* __restrict__ count
   if (threadIdx.x == 1023) offsets[n] = part[1023];
 }
 
-// Pass 2: ascending light indices per tile at offsets[tile].
-__global__ __launch_bounds__(kCullBlock) void kCullFill(Frustum4 f, const float4* __restrict__ lights,
-                                                        uint32_t numLights, uint32_t ngx, uint32_t ngy,
-                                                        const uint32_t* __restrict__ offsets,
+// Pass 3: one wave per tile writes the ascending light indices of its mask at offsets[tile], the
+// lists truncated at `capacity` entries in all. Lane l expands mask word l of each round of 64 words,
+// at the position a wave prefix sum of the words' population counts gives it.
+__global__ __launch_bounds__(kCullBlock) void kCullEmit(const uint64_t* __restrict__ masks, uint32_t groups,
+                                                        uint32_t ntiles, const uint32_t* __restrict__ offsets,
                                                         uint32_t* __restrict__ out, uint32_t capacity) {
-  const uint32_t tile = blockIdx.x;
-  const uint32_t tx = tile % ngx, ty = tile / ngx;
-  float pl[16];
-  tilePlanes(f, ngx, ngy, tx, ty, pl);
-  __shared__ uint32_t waveCount[kCullBlock / 64];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t tile = blockIdx.x * (kCullBlock / 64u) + (threadIdx.x >> 6);
+  if (tile >= ntiles) return;
+  const uint64_t* row = masks + size_t(tile) * groups;
   uint32_t base = offsets[tile];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t c = 0; c < numLights; c += kCullBlock) {
-    const uint32_t i = c + threadIdx.x;
-    const bool in = i < numLights && sphereInFrustum(pl, lights[i]);
-    const uint64_t m = __ballot(in);
-    if (lane == 0) waveCount[wave] = __popcll(m);
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (uint32_t w = 0; w < kCullBlock / 64; ++w) {
-      if (w < wave) before += waveCount[w];
-      total += waveCount[w];
+  for (uint32_t g0 = 0; g0 < groups; g0 += 64u) {
+    const uint32_t g = g0 + lane;
+    uint64_t m = g < groups ? row[g] : 0ull;
+    const uint32_t c = uint32_t(__popcll(m));
+    uint32_t incl = c;  // inclusive prefix sum over the lanes
+    for (uint32_t o = 1; o < 64u; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
     }
-    if (in) {
-      const uint32_t pos = base + before + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
-      if (pos < capacity) out[pos] = i;
+    uint32_t pos = base + incl - c;
+    while (m) {
+      const uint32_t b = uint32_t(__builtin_ctzll(m));
+      m &= m - 1ull;
+      if (pos < capacity) out[pos] = g * 64u + b;
+      ++pos;
     }
-    base += total;
-    __syncthreads();
+    base += __shfl(incl, 63, 64);
   }
 }
 

@@ -18,6 +18,7 @@
 
 #include "../../../include/rtq.h"
 #include "scene_layout.h"
+#include "stack_class.h"
 #include "kernels.h"
 #include "layout.h"
 
@@ -90,21 +91,23 @@ void freeScene(rtq_scene* s) {
   delete s;
 }
 
-// Smallest per-lane LDS stack (entries) that holds the deepest path of a scene (a walk pushes at most
-// one entry per inner node on its path, so a tree whose leaves lie at depth <= d needs d entries): a
-// smaller stack leaves room for more resident waves per CU. With 22 entries a 128-lane block takes
-// 11 KB of LDS, and the grid kernels reach their register limit of 7 waves per SIMD; with 24, 6.5.
-int stackClass(uint32_t depth) { return depth <= 22 ? 22 : (depth <= 24 ? 24 : (depth <= 32 ? 32 : 64)); }
-
+// Instantiates f with the LDS stack class of a tree of this depth (stack_class.h); false, with
+// nothing launched, for a tree deeper than any class holds.
 template <class F>
-void withStack(uint32_t depth, F&& f) {
-  switch (stackClass(depth)) {
-    case 22: f(std::integral_constant<int, 22>{}); break;
-    case 24: f(std::integral_constant<int, 24>{}); break;
-    case 32: f(std::integral_constant<int, 32>{}); break;
-    default: f(std::integral_constant<int, 64>{}); break;
+bool withStack(uint32_t depth, F&& f) {
+  switch (rtq::gridStackClass(depth)) {
+    case 22: f(std::integral_constant<int, 22>{}); return true;
+    case 24: f(std::integral_constant<int, 24>{}); return true;
+    case 32: f(std::integral_constant<int, 32>{}); return true;
+    case 64: f(std::integral_constant<int, 64>{}); return true;
   }
+  return false;
 }
+
+#define RTQ_WITH_STACK(depth, ...)                                                       \
+  do {                                                                                   \
+    if (!withStack((depth), __VA_ARGS__)) return fail(RTQ_ERR_LIMIT, "tree deeper than 64"); \
+  } while (0)
 
 }  // namespace
 
@@ -453,7 +456,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   if (!counters && s->dev->persistent == 2) {
     const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
     const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
-    withStack(s->info.max_depth, [&](auto S) {
+    RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
       constexpr int K = decltype(S)::value;
       if (flags == RTQ_TRACE_CLOSEST)
         hipLaunchKernelGGL((rtq::kTraceWaveRange<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid, perm,
@@ -470,7 +473,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
     hipLaunchKernelGGL(rtq::kResetCounter, dim3(1), dim3(64), 0, st, counter);
     const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
     const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
-    withStack(s->info.max_depth, [&](auto S) {
+    RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
       constexpr int K = decltype(S)::value;
       if (flags == RTQ_TRACE_CLOSEST)
         hipLaunchKernelGGL((rtq::kTracePersistent<K, false>), pgrid, block, 0, st, s->view, r, n, out, nid, counter,
@@ -482,7 +485,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
     RTQ_HIP(hipGetLastError());
     return perm ? sortRelease(s->dev, st) : RTQ_OK;
   }
-  withStack(s->info.max_depth, [&](auto S) {
+  RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
     constexpr int K = decltype(S)::value;
     if (flags == RTQ_TRACE_CLOSEST) {
       float4* h = static_cast<float4*>(out);
@@ -531,7 +534,7 @@ rtq_status rtq_trace_camera(rtq_scene* s, const rtq_frustum* f, uint32_t width, 
   hipStream_t st = static_cast<hipStream_t>(stream);
   float4* h = reinterpret_cast<float4*>(d_hits);
   const uint32_t ref = flags & RTQ_CAMERA_REFERENCE_ROWS;
-  withStack(s->info.max_depth, [&](auto S) {
+  RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
     constexpr int K = decltype(S)::value;
     hipLaunchKernelGGL((rtq::kTraceCamera<K>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, fr, width,
                        height, ref, h, d_leaf_nid);
@@ -564,7 +567,7 @@ rtq_status rtq_trace_camera_record(rtq_scene* s, const rtq_frustum* f, uint32_t 
   hipStream_t st = static_cast<hipStream_t>(stream);
   uint4* rec = reinterpret_cast<uint4*>(d_records);
   const uint32_t ref = flags & RTQ_CAMERA_REFERENCE_ROWS;
-  withStack(s->info.max_depth, [&](auto S) {
+  RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
     constexpr int K = decltype(S)::value;
     hipLaunchKernelGGL((rtq::kTraceRecord<K>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, fr, width, height,
                        ref, sd[0], sd[1], sd[2], shadowFlags, h, lpf ? 1u : 0u, rec);
@@ -593,7 +596,7 @@ rtq_status rtq_trace_probe_field(rtq_scene* s, const rtq_lpf* lpf, const float d
   const uint64_t waves = ((probes + 63u) / 64u) * rtq::kProbeRays;
   const uint32_t blocks = uint32_t((waves + 1u) / 2u);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  withStack(s->info.max_depth, [&](auto S) {
+  RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
     constexpr int K = decltype(S)::value;
     hipLaunchKernelGGL((rtq::kTraceProbes<K>), dim3(blocks), dim3(rtq::kTraceBlock), 0, st, s->view, h, pd,
                        uint32_t(probes), sd[0], sd[1], sd[2], shadowFlags, reinterpret_cast<uint4*>(d_records));
@@ -615,7 +618,7 @@ rtq_status rtq_shadow_masks(rtq_scene* s, const rtq_ray* d_rays, const rtq_hit* 
   const dim3 grid(uint32_t((uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock)), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(d_rays);
   const float4* h = reinterpret_cast<const float4*>(d_hits);
-  withStack(s->info.max_depth, [&](auto S) {
+  RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
     constexpr int K = decltype(S)::value;
     hipLaunchKernelGGL((rtq::kShadowMasks<K>), grid, block, 0, st, s->view, r, h, d_leaf_nid, n, sd[0], sd[1], sd[2],
                        flags, d_masks);
@@ -645,34 +648,65 @@ rtq_status rtq_selftest_division(rtq_device* dev, uint64_t seed, uint64_t count,
   return RTQ_OK;
 }
 
+namespace {
+
+// Checks a culling call and fills the launch geometry; RTQ_OK or the error to return.
+rtq_status cullSetup(rtq_scene* s, const rtq_frustum* f, uint32_t width, uint32_t height, const uint32_t* d_tile_offsets,
+                     const uint32_t* d_tile_lights, uint32_t capacity, rtq::Frustum4& fr, uint32_t& ngx, uint32_t& ngy,
+                     uint32_t& groups) {
+  if (!s || !f || !d_tile_offsets || (!d_tile_lights && capacity)) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (width == 0 || height == 0) return fail(RTQ_ERR_INVALID_ARGUMENT, "empty frame");
+  ngx = (width + 15u) / 16u;
+  ngy = (height + 15u) / 16u;
+  if (uint64_t(ngx) * ngy > (1u << 24)) return fail(RTQ_ERR_LIMIT, "more than 2^24 tiles");
+  groups = (s->view.numLights + 63u) / 64u;
+  std::memcpy(&fr, f, sizeof(fr));
+  return RTQ_OK;
+}
+
+// kCullMask -> kScan -> kCullEmit on `st` with the scratch at `scratch` (cullScratchWords words).
+hipError_t cullLaunch(rtq_scene* s, const rtq::Frustum4& fr, uint32_t ngx, uint32_t ngy, uint32_t groups,
+                      uint32_t* scratch, uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity,
+                      hipStream_t st) {
+  const uint32_t ntiles = ngx * ngy;
+  uint64_t* masks = reinterpret_cast<uint64_t*>(scratch);
+  uint32_t* counts = scratch + 2ull * ntiles * groups;
+  const dim3 grid((ntiles + rtq::kCullBlock / 64 - 1) / (rtq::kCullBlock / 64)), block(rtq::kCullBlock);
+  const float4* lights = reinterpret_cast<const float4*>(s->view.lights);
+  hipLaunchKernelGGL(rtq::kCullMask, grid, block, 0, st, fr, lights, s->view.numLights, ngx, ngy, groups, masks, counts);
+  hipLaunchKernelGGL(rtq::kScan, dim3(1), dim3(1024), 0, st, counts, ntiles, d_tile_offsets);
+  hipLaunchKernelGGL(rtq::kCullEmit, grid, block, 0, st, masks, groups, ntiles, d_tile_offsets, d_tile_lights, capacity);
+  return hipGetLastError();
+}
+
+}  // namespace
+
 rtq_status rtq_cull_lights(rtq_scene* s, const rtq_frustum* f, uint32_t width, uint32_t height,
                            uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity, uint32_t* h_total,
                            void* stream) {
-  if (!s || !f || !d_tile_offsets || (!d_tile_lights && capacity)) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (width == 0 || height == 0) return fail(RTQ_ERR_INVALID_ARGUMENT, "empty frame");
-  const uint32_t ngx = (width + 15u) / 16u, ngy = (height + 15u) / 16u;
-  const uint64_t ntiles64 = uint64_t(ngx) * ngy;
-  if (ntiles64 > (1u << 24)) return fail(RTQ_ERR_LIMIT, "more than 2^24 tiles");
-  const uint32_t ntiles = uint32_t(ntiles64);
+  rtq::Frustum4 fr;
+  uint32_t ngx = 0, ngy = 0, groups = 0;
+  rtq_status cs = cullSetup(s, f, width, height, d_tile_offsets, d_tile_lights, capacity, fr, ngx, ngy, groups);
+  if (cs != RTQ_OK) return cs;
+  const uint32_t ntiles = ngx * ngy;
   RTQ_HIP(hipSetDevice(s->dev->ordinal));
   hipStream_t st = static_cast<hipStream_t>(stream);
-  rtq::Frustum4 fr;
-  std::memcpy(&fr, f, sizeof(fr));
-  const float4* lights = reinterpret_cast<const float4*>(s->view.lights);
-  uint32_t* counts = nullptr;
-  RTQ_HIP(hipMallocAsync(reinterpret_cast<void**>(&counts), sizeof(uint32_t) * ntiles, st));
-  hipLaunchKernelGGL(rtq::kCullCount, dim3(ntiles), dim3(rtq::kCullBlock), 0, st, fr, lights, s->view.numLights, ngx,
-                     ngy, counts);
-  hipLaunchKernelGGL(rtq::kScan, dim3(1), dim3(1024), 0, st, counts, ntiles, d_tile_offsets);
-  hipLaunchKernelGGL(rtq::kCullFill, dim3(ntiles), dim3(rtq::kCullBlock), 0, st, fr, lights, s->view.numLights, ngx,
-                     ngy, d_tile_offsets, d_tile_lights, capacity);
-  hipError_t le = hipGetLastError();
+  // stream-ordered scratch: the call stays asynchronous
+  uint32_t* scratch = nullptr;
+  RTQ_HIP(hipMallocAsync(reinterpret_cast<void**>(&scratch),
+                         sizeof(uint32_t) * std::max<uint64_t>(1, rtq::cullScratchWords(ntiles, groups)), st));
+  hipError_t le = cullLaunch(s, fr, ngx, ngy, groups, scratch, d_tile_offsets, d_tile_lights, capacity, st);
+  hipError_t fe = hipFreeAsync(scratch, st);
+  if (le != hipSuccess) {
+    (void)hipStreamSynchronize(st);
+    return hipFail(le, "culling kernels");
+  }
+  if (fe != hipSuccess) return hipFail(fe, "hipFreeAsync");
+  if (!h_total) return RTQ_OK;
   uint32_t total = 0;
-  (void)hipMemcpyAsync(&total, d_tile_offsets + ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
-  (void)hipFreeAsync(counts, st);
+  RTQ_HIP(hipMemcpyAsync(&total, d_tile_offsets + ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   RTQ_HIP(hipStreamSynchronize(st));
-  if (le != hipSuccess) return hipFail(le, "culling kernels");
-  if (h_total) *h_total = total;
+  *h_total = total;
   if (total > capacity) return fail(RTQ_ERR_LIMIT, "tile light lists exceed capacity");
   return RTQ_OK;
 }

@@ -216,13 +216,55 @@ class Batch:
         return p, ms.value
 
 
+def check_request_buffers(flags: int, rays: np.ndarray, out: np.ndarray, nid: np.ndarray | None) -> int:
+    """Validates a served request's host buffers before their addresses reach the library, which reads
+    32 B per ray and writes 16 B (closest) or 4 B (any) per ray and 4 B per leaf id: rays must be a
+    contiguous RAY_DTYPE array, out a contiguous HIT_DTYPE (closest) or uint32 (any) array and nid, if
+    given, a contiguous uint32 array, both of at least as many entries as there are rays. Returns the
+    ray count; raises ValueError otherwise."""
+    if flags not in (B.RTQ_TRACE_CLOSEST, B.RTQ_TRACE_ANY):
+        raise ValueError(f"bad flags {flags}")
+    if not isinstance(rays, np.ndarray) or rays.dtype != B.RAY_DTYPE or rays.ndim != 1 or \
+            not rays.flags["C_CONTIGUOUS"]:
+        raise ValueError("rays must be a contiguous 1-D array of RAY_DTYPE records")
+    n = rays.size
+    if n == 0:
+        raise ValueError("empty request")
+    want = B.HIT_DTYPE if flags == B.RTQ_TRACE_CLOSEST else np.dtype(np.uint32)
+    if not isinstance(out, np.ndarray) or out.dtype != want or not out.flags["C_CONTIGUOUS"] or \
+            not out.flags["WRITEABLE"] or out.size < n:
+        raise ValueError(f"out must be a writeable contiguous {want} array of at least {n} entries")
+    if nid is not None:
+        if flags != B.RTQ_TRACE_CLOSEST:
+            raise ValueError("leaf node ids exist for closest-hit requests only")
+        if not isinstance(nid, np.ndarray) or nid.dtype != np.uint32 or not nid.flags["C_CONTIGUOUS"] or \
+                not nid.flags["WRITEABLE"] or nid.size < n:
+            raise ValueError(f"nid must be a writeable contiguous uint32 array of at least {n} entries")
+    return n
+
+
 class Request:
+    """A submitted request. The library reads the rays and writes the results asynchronously, so the
+    request keeps its host buffers alive; a request dropped before it finishes is cancelled if it is
+    still queued, otherwise waited for, and released before those buffers can go."""
+
     def __init__(self, server: "Server", handle, rays, out, nid):
         self.server = server
         self.handle = handle
         self._keep = (rays, out, nid)  # host buffers must outlive the request
         self.out = out
         self.nid = nid
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib = self.server.dev.lib
+                if lib.rtq_request_cancel(self.handle) != B.RTQ_OK:
+                    lib.rtq_request_wait(self.handle, 0xFFFFFFFF)
+                lib.rtq_request_release(self.handle)
+                self.handle = None
+        except Exception:
+            pass
 
     def wait(self, timeout_ms: int = 0xFFFFFFFF) -> int:
         return self.server.dev.lib.rtq_request_wait(self.handle, timeout_ms)
@@ -259,11 +301,11 @@ class Server:
     def submit(self, scene: int, flags: int, rays: np.ndarray, out: np.ndarray | None = None,
                nid: np.ndarray | None = None, with_nid: bool = True) -> Request:
         n = rays.size
-        rays = np.ascontiguousarray(rays)
         if out is None:
             out = np.zeros(n, B.HIT_DTYPE) if flags == B.RTQ_TRACE_CLOSEST else np.zeros(n, np.uint32)
         if nid is None and flags == B.RTQ_TRACE_CLOSEST and with_nid:
             nid = np.zeros(n, np.uint32)
+        n = check_request_buffers(flags, rays, out, nid)
         h = C.c_void_p()
         B.check_device("rtq_server_submit",
                        self.dev.lib.rtq_server_submit(self.handle, scene, flags, rays.ctypes.data, n, out.ctypes.data,

@@ -235,9 +235,11 @@ rtq_status rtq_shadow_masks(rtq_scene* scene, const rtq_ray* d_rays, const rtq_h
 /* Per-tile frustum light culling: the light part of cullWithFrustumTile (Shaders/cullFrustum.glsl:
  * 63-100,136-153) over a ceil(width/16) x ceil(height/16) tile grid, tile (x, y) = id y*tiles_x + x.
  * Output is CSR: d_tile_offsets[num_tiles+1], d_tile_lights[...] ascending light indices per tile
- * (the reference's 16-entry atomic-order list is the first 16 of it). If `capacity` is smaller
- * than the total, lists are truncated at capacity and RTQ_ERR_LIMIT is returned; *h_total (may be
- * NULL) receives the full total. Synchronises `stream`. */
+ * (the reference's 16-entry atomic-order list is the first 16 of it); d_tile_offsets[num_tiles] is
+ * the total, and lists beyond `capacity` entries in all are truncated, never overrun. Each light is
+ * tested once per tile. With h_total NULL the call is asynchronous on `stream` (its scratch is
+ * stream-ordered device memory). With h_total set it synchronises `stream`, stores the total there
+ * and returns RTQ_ERR_LIMIT if the total exceeds `capacity`. */
 rtq_status rtq_cull_lights(rtq_scene* scene, const rtq_frustum* frustum, uint32_t width, uint32_t height,
                            uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity, uint32_t* h_total,
                            void* stream);

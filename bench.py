@@ -10,11 +10,16 @@ A step is one closest-hit dispatch over all 2 073 600 primary rays with rays, BV
 resident in HBM. `value` is primary rays / step time. The shadow pass is timed separately and
 reported beside it.
 
-roofline: algorithmic bytes per ray B = 32 (ray) + 16 (hit) + 64*N_inner + 4*N_leaf + 48*N_tri, with the
-per-ray counts from the counting variant of the same kernel (tests check those counts equal the
-oracle's); achieved = sum(B) / average kernel time, peak = 8 TB/s HBM.
+roofline: the traversal kernels are held by VALU instruction issue (the scene stays in the Infinity
+Cache, so HBM sees about 1 % of the bytes the rays read): achieved = the kernel's SQ_INSTS_VALU per
+launch (committed PMC summary, profiles/pmc_<workload>.json) / the launch time measured here with HIP
+events, peak = 614.4 G wave64 VALU instructions/s; traffic = HBM bytes per launch from FETCH_SIZE /
+WRITE_SIZE and hbm_frac its share of 8 TB/s. The reference-layout algorithmic bytes per ray,
+B = 32 (ray) + 16 (hit) + 64*N_inner + 4*N_leaf + 48*N_tri, with per-ray counts from the counting
+variant of the same kernel (tests check those counts equal the oracle's), are reported beside it.
 cpu_baseline: the scalar oracle (a restatement of the reference traversal) on a bounded sample of
-the same rays, on the host cores.
+the same rays, on the host cores. The same leg checks every ray of the timed launch against the
+oracle (parity) and states the FP64 error of its hits (fp64_error).
 """
 from __future__ import annotations
 
@@ -117,20 +122,70 @@ def kernel_name(args, depth: int, any_hit: bool = False) -> str:
     return f"{'kTraceAny' if any_hit else 'kTraceClosest'}<{kernel_stack(depth)},false>"
 
 
-def measured_traffic(workload: str, kernel: str):
-    """HBM bytes per launch from the committed PMC summary of this workload and kernel, if present:
-    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the same counters read for a copy of
-    known size."""
+# Published MI355X figures: 8 TB/s HBM3E; 256 CUs at 2.4 GHz peak engine clock. A wave64 VALU
+# instruction occupies a 16-lane SIMD for 4 cycles and a CU has 4 SIMDs, so the chip issues at most
+# 256 x 2.4e9 = 614.4 G wave64 VALU instructions per second (the published 157.3 TFLOP/s fp32 figure
+# / 2 for packed / 2 for FMA / 64 lanes is the same number).
+VALU_PEAK_GINSTR = 256 * 2.4
+
+
+def pmc_record(workload: str):
+    """The committed PMC summary of a workload (profiles/pmc_<workload>.json), or None."""
     path = os.path.join(ROOT, "profiles", f"pmc_{workload}.json")
     if not os.path.exists(path):
         return None, None
     with open(path) as fh:
-        rec = json.load(fh)
-    k = rec.get("kernels", {}).get(kernel)
-    if not k:
+        return json.load(fh), os.path.relpath(path, ROOT)
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch from the committed PMC summary of this workload and kernel, if present:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the same counters read for a copy of
+    known size."""
+    rec, src = pmc_record(workload)
+    k = (rec or {}).get("kernels", {}).get(kernel)
+    if not k or "fetch_kib" not in k:
         return None, None
     b = 1024.0 * (k["fetch_kib"] * rec["fetch_scale"] + k["write_kib"] * rec["write_scale"])
-    return int(b), os.path.relpath(path, ROOT)
+    return int(b), src
+
+
+def roofline(workload: str, kernel: str, ms: float, algorithmic: dict | None = None) -> dict:
+    """The kernel's position under its two roofs, from the committed PMC summary of the same workload
+    and kernel and the launch time measured here: VALU issue (SQ_INSTS_VALU per launch / launch time
+    against 614.4 G wave64 instructions/s) and HBM (traffic = HBM bytes per launch from FETCH_SIZE /
+    WRITE_SIZE, / launch time against 8 TB/s). bound/achieved/peak/frac name the roof with the higher
+    fraction. `algorithmic` (the reference-layout bytes per launch) is informational: the caches serve
+    most of those bytes, so they are not a fraction of any roof."""
+    rec, src = pmc_record(workload)
+    k = (rec or {}).get("kernels", {}).get(kernel) or {}
+    valu = k.get("sq_insts_valu")
+    traffic, _ = measured_traffic(workload, kernel)
+    sec = ms * 1e-3
+    valu_rate = valu / sec / 1e9 if valu else None
+    hbm_rate = traffic / sec / 1e9 if traffic else None
+    valu_frac = valu_rate / VALU_PEAK_GINSTR if valu_rate else None
+    hbm_frac = hbm_rate / HBM_PEAK_GBS if hbm_rate else None
+    use_hbm = hbm_frac is not None and (valu_frac is None or hbm_frac > valu_frac)
+    out = {
+        "bound": "hbm" if use_hbm else "valu_issue",
+        "achieved": round(hbm_rate if use_hbm else valu_rate, 1) if (hbm_rate if use_hbm else valu_rate) else None,
+        "peak": HBM_PEAK_GBS if use_hbm else VALU_PEAK_GINSTR,
+        "unit": "GB/s" if use_hbm else "G wave64 VALU instr/s",
+        "frac": round(hbm_frac if use_hbm else valu_frac, 4) if (hbm_frac if use_hbm else valu_frac) else None,
+        "traffic": traffic,
+        "valu_frac": round(valu_frac, 4) if valu_frac else None,
+        "hbm_frac": round(hbm_frac, 4) if hbm_frac else None,
+        "valu_instructions_per_launch": valu,
+        "valu_g_per_s": round(valu_rate, 1) if valu_rate else None,
+        "hbm_gb_s": round(hbm_rate, 1) if hbm_rate else None,
+        "active_lanes_per_valu_instr": k.get("active_lane_frac"),
+        "counter_source": src if k else None,
+        "kernel": kernel,
+    }
+    if algorithmic is not None:
+        out["algorithmic"] = algorithmic
+    return out
 
 
 def host_threads(req: int) -> int:
@@ -143,46 +198,116 @@ def host_threads(req: int) -> int:
     return max(1, min(16, n))
 
 
-def cpu_baseline(blob, ranges, pos, rays, threads: int, target_s: float):
-    import oracle  # the CPU baseline leg is the only place bench.py touches the oracle
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    sc = oracle.Scene(blob, ranges, pos)
+
+def timed_oracle(fn, rays, threads: int, target_s: float, what: str):
+    """Repeats fn(sample, threads) over a strided sample of `rays` sized for about target_s of wall
+    time; -> (Mrays/s, description)."""
     probe = rays[:: max(1, rays.size // 20000)][:20000]
     t0 = time.perf_counter()
-    sc.trace_closest(probe, threads=threads)
-    dt = time.perf_counter() - t0
-    rate = probe.size / max(dt, 1e-6)
+    fn(np.ascontiguousarray(probe), threads)
+    rate = probe.size / max(time.perf_counter() - t0, 1e-6)
     n = int(min(rays.size, max(probe.size, rate * target_s)))
     stride = max(1, rays.size // n)
     sample = np.ascontiguousarray(rays[::stride][:n])
-    # whole frames are repeated until the sample has taken about target_s of wall time
     passes, traced = 0, 0
     t0 = time.perf_counter()
     while True:
-        sc.trace_closest(sample, threads=threads)
+        fn(sample, threads)
         passes += 1
         traced += sample.size
         dt = time.perf_counter() - t0
         if dt >= target_s or passes >= 1000:
             break
-    out = {
-        "value": round(traced / dt / 1e6, 4),
-        "unit": "Mrays/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"{sample.size} of the {rays.size} primary rays (stride {stride}) x {passes} passes, closest "
-                  f"hit, bitstack restatement of traverseBvh, {threads} threads, {dt:.1f} s",
-    }
-    if threads > 1:
-        # the same restatement on one thread, over a strided sample sized for about a fifth of target_s
-        one_n = int(min(rays.size, max(1000, traced / dt / threads * target_s / 5)))
-        one = np.ascontiguousarray(rays[:: max(1, rays.size // one_n)][:one_n])
+    return traced / dt / 1e6, (f"{sample.size} of the {rays.size} {what} (stride {stride}) x {passes} passes, "
+                               f"{threads} threads, {dt:.1f} s")
+
+
+def check_against_oracle(sc, rays, g_hits, g_nid, g_masks, threads: int) -> tuple[dict, dict]:
+    """Every ray of the timed launch against the oracle (a restatement of the reference traversal):
+    closest-hit triangle id, leaf node id and the bits of t, u, v; the shadow masks of the timed
+    shadow pass; and the FP64 error of the fp32 hits (t relative to the FP64 intersection of the same
+    triangle, after the reference's t - 1e-5 shrink; u, v absolute)."""
+    o = sc.trace_closest(rays, threads=threads)
+    bad = (g_hits["tri_id"] != o["tri_id"]) | (g_nid != o["nid"])
+    for k in ("t", "u", "v"):
+        bad |= g_hits[k].view(np.uint32) != o[k].view(np.uint32)
+    om = sc.shadow_masks(rays, o["t"], o["nid"], np.zeros(3, np.float32), 0, threads=threads)
+    hit = o["tri_id"] != 0xFFFFFFFF
+    parity = {"rays_checked": int(rays.size), "mismatches": int(bad.sum()),
+              "shadow_checked": int(rays.size), "shadow_mismatches": int((g_masks != om).sum()),
+              "hits": int(hit.sum()), "near_tie_exempted": 0,
+              "checker": "oracle/rtq_oracle.c bitstack traverseBvh + computeShadowMask, every ray of the timed buffer"}
+    t64, u64, v64 = sc.hits_f64(rays, o["tri_id"])
+    h = hit & np.isfinite(t64)
+    t_raw = o["t"][h].astype(np.float64) + np.float64(np.float32(1e-5))
+    rel = np.abs(t_raw - t64[h]) / np.abs(t64[h])
+    du = np.abs(o["u"][h].astype(np.float64) - u64[h])
+    dv = np.abs(o["v"][h].astype(np.float64) - v64[h])
+    err = {"hits": int(h.sum()),
+           "t_max_rel": float(rel.max()) if h.any() else None, "t_p99_rel": float(np.percentile(rel, 99)) if h.any() else None,
+           "u_max_abs": float(du.max()) if h.any() else None, "v_max_abs": float(dv.max()) if h.any() else None,
+           "u_p99_abs": float(np.percentile(du, 99)) if h.any() else None,
+           "v_p99_abs": float(np.percentile(dv, 99)) if h.any() else None,
+           "oracle": "orc_hits_f64: double Moller-Trumbore on the hit triangle; t compared as stored t + 1e-5f"}
+    return parity, err
+
+
+def cpu_config1(threads: int) -> dict:
+    """The CPU method on the closed box (16 384 triangles): the builder restatement's wall time and the
+    oracle's scalar closest-hit traversal of 256x256 jittered primary rays, on 1 thread and on `threads`."""
+    import oracle
+    from tim_raytracing_amd import host as H
+
+    out = {}
+    for th in sorted({1, threads}):
         t0 = time.perf_counter()
-        sc.trace_closest(one, threads=1)
-        dt1 = time.perf_counter() - t0
-        out["single_thread_value"] = round(one.size / dt1 / 1e6, 4)
-        out["single_thread_sample"] = f"{one.size} primary rays, 1 thread, {dt1:.1f} s"
+        b, f, w, h = H.workload(1, threads=th)
+        build_s = time.perf_counter() - t0
+        blob, ranges = b.fill_gpu_buffer()
+        sc = oracle.Scene(blob, ranges, b.positions())
+        rays = H.frustum_rays(f, w, h, seed=1)
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            sc.trace_closest(rays, threads=th)
+            reps += 1
+            if time.perf_counter() - t0 > 1.0:
+                break
+        dt = time.perf_counter() - t0
+        out[f"threads_{th}"] = {"build_s": round(build_s, 4), "trace_mrays_s": round(reps * rays.size / dt / 1e6, 3),
+                                "rays": int(rays.size), "passes": reps}
+    out["workload"] = "box-16K-primary-256x256 (16 384 tris, 256x256 jittered primary rays, t limit 100)"
     return out
+
+
+def cpu_baseline(blob, ranges, pos, rays, threads: int, target_s: float, gpu=None):
+    import oracle  # the CPU baseline leg is the only place bench.py touches the oracle
+
+    sc = oracle.Scene(blob, ranges, pos)
+    extra = {}
+    if gpu is not None:
+        extra["parity"], extra["fp64_error"] = check_against_oracle(sc, rays, *gpu, threads=threads)
+    value, desc = timed_oracle(lambda r, th: sc.trace_closest(r, threads=th), rays, threads, target_s,
+                               "primary rays, closest hit, bitstack restatement of traverseBvh")
+    out = {"value": round(value, 4), "unit": "Mrays/s", "cores": threads, "kind": "port", "sample": desc,
+           "cpu_model": cpu_model()}
+    if threads > 1:
+        one, one_desc = timed_oracle(lambda r, th: sc.trace_closest(r, threads=th), rays, 1, target_s / 5,
+                                     "primary rays")
+        out["single_thread_value"] = round(one, 4)
+        out["single_thread_sample"] = one_desc
+    out["config1_cpu_method"] = cpu_config1(threads)
+    return out, extra
 
 
 def main():
@@ -223,7 +348,6 @@ def main():
     scene = D.Scene(dev, blob, ranges, pos)
     info = scene.info()
     kname = kernel_name(args, int(info.max_depth))
-    traffic, traffic_src = measured_traffic("spheres-1M-primary-1920x1080", f"{kname}@{args.ray_order}")
     stream = dev.stream()
     sptr = stream.ptr
     d_rays = dev.upload(rays)
@@ -255,7 +379,8 @@ def main():
     d_cnt.free()
 
     ms = timed(closest, args.warmup, args.steps)
-    hits = d_hits.read(B.HIT_DTYPE, n)
+    hits = d_hits.read(B.HIT_DTYPE, n)  # the timed launch's records (its last step)
+    nids = d_nid.read(np.uint32, n)
     hit_mask = hits["tri_id"] != B.RTQ_NO_HIT
     n_hit = int(hit_mask.sum())
 
@@ -297,7 +422,22 @@ def main():
     d_rays.write(rays_other)
     other_ms = timed(closest, max(1, args.warmup // 2), args.steps)
 
-    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    alg_gbs = alg_bytes / (ms * 1e-3) / 1e9
+    algorithmic = {
+        "bytes_per_launch": int(alg_bytes),
+        "achieved_gb_s": round(alg_gbs, 1),
+        "vs_hbm_peak": round(alg_gbs / HBM_PEAK_GBS, 4),
+        "mean_bytes_per_ray": round(alg_bytes / n, 1),
+        "mean_inner_nodes": round(sum_inner / n, 2),
+        "mean_leaves": round(sum_leaf / n, 3),
+        "mean_triangle_tests": round(sum_tri / n, 2),
+        "hbm_roofline_mrays_s": round(HBM_PEAK_GBS * 1e9 / (alg_bytes / n) / 1e6, 1),
+        # reported, not a roof: about 50 flops per inner node (two slab tests) and 45 per triangle test
+        "flops_per_ray": round((50.0 * sum_inner + 45.0 * sum_tri) / n, 1),
+        "note": "B = 32 + 16 + 64 N_inner + 4 N_leaf + 48 N_tri per ray at the reference layout, counts from the "
+                "counting kernel (tests check them against the oracle); the 119 MB scene stays in the 256 MB "
+                "Infinity Cache, so HBM sees about 1 % of these bytes and they are not a fraction of the HBM roof",
+    }
     result = {
         "metric": "closest-hit Mrays/s @1M tris (frac of HBM roofline)",
         "value": round(n / (ms * 1e-3) / 1e6, 2),
@@ -322,28 +462,8 @@ def main():
             "bvh_max_depth": int(info.max_depth),
             "scene_device_bytes": int(info.device_bytes),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "traffic_source": traffic_src,
-            "kernel": kname,
-            "dispatch": dispatch_cfg,
-            "algorithmic_bytes_per_launch": int(alg_bytes),
-            "mean_bytes_per_ray": round(alg_bytes / n, 1),
-            "mean_inner_nodes": round(sum_inner / n, 2),
-            "mean_leaves": round(sum_leaf / n, 3),
-            "mean_triangle_tests": round(sum_tri / n, 2),
-            "roofline_mrays_s": round(HBM_PEAK_GBS * 1e9 / (alg_bytes / n) / 1e6, 1),
-            # reported, not a roof: about 50 flops per inner node (two slab tests) and 45 per triangle test
-            "flops_per_ray": round((50.0 * sum_inner + 45.0 * sum_tri) / n, 1),
-            "note": "achieved counts every node and triangle record a ray reads at the reference layout; "
-                    "the caches serve most of them (traffic is what reaches HBM), so frac can exceed 1; "
-                    "the kernel itself is held by instruction issue and load latency",
-        },
+        "roofline": dict(roofline("spheres-1M-primary-1920x1080", f"{kname}@{args.ray_order}", ms, algorithmic),
+                         dispatch=dispatch_cfg),
         "hit_rate": round(n_hit / n, 4),
         "shadow": {
             "rays": n_hit,
@@ -364,7 +484,9 @@ def main():
         "host": {"threads": threads, "bvh_build_s": round(build_s, 2), "setup_s": round(setup_s, 2)},
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(blob, ranges, pos, rays, threads, args.cpu_seconds)
+        result["cpu_baseline"], checks = cpu_baseline(blob, ranges, pos, rays, threads, args.cpu_seconds,
+                                                      gpu=(hits, nids, masks))
+        result.update(checks)
     print(json.dumps(result), flush=True)
 
 
@@ -657,9 +779,7 @@ def bench_incoherent(args):
     d_h = dev.alloc(16 * n)
     d_n = dev.alloc(4 * n)
 
-    def trace(rays):
-        d_r = dev.upload(rays)
-        fn = lambda: scene.dispatch(d_r.ptr, n, B.RTQ_TRACE_CLOSEST, d_h.ptr, d_n.ptr, sptr)
+    def timed(fn):
         for _ in range(args.warmup):
             fn()
         stream.synchronize()
@@ -667,32 +787,70 @@ def bench_incoherent(args):
         for _ in range(args.steps):
             fn()
         ev1.record(sptr)
-        ms = ev0.elapsed_ms(ev1) / args.steps
-        return ms, d_h.read(B.HIT_DTYPE, n)
+        return ev0.elapsed_ms(ev1) / args.steps
 
-    ms1, hits1 = trace(rays1)
+    def trace(rays):
+        d_r = dev.upload(rays)
+        ms = timed(lambda: scene.dispatch(d_r.ptr, n, B.RTQ_TRACE_CLOSEST, d_h.ptr, d_n.ptr, sptr))
+        return ms, d_h.read(B.HIT_DTYPE, n), d_n.read(np.uint32, n)
+
+    ms1, hits1, nid1 = trace(rays1)
     rays2 = b.bounce_rays(4, rays1, hits1)
-    ms2, hits2 = trace(rays2)
+    ms2, hits2, nid2 = trace(rays2)
     ntiles = ((W + 15) // 16) * ((Hh + 15) // 16)
     offs, lists = scene.cull_lights(f, W, Hh)
-    t = time.perf_counter()
-    reps = max(1, args.steps)
-    for _ in range(reps):
-        scene.cull_lights(f, W, Hh, capacity=int(lists.size))
-    cull_ms = (time.perf_counter() - t) * 1e3 / reps
-    print(json.dumps({
+    # culling on the stream alone (asynchronous entry point, no host synchronisation in the timed loop)
+    d_off, d_lists = dev.alloc(4 * (ntiles + 1)), dev.alloc(4 * max(1, lists.size))
+    cull_ms = timed(lambda: scene.cull_lights_async(f, W, Hh, d_off.ptr, d_lists.ptr, int(lists.size), sptr))
+    g_off, g_lists = d_off.read(np.uint32, ntiles + 1), d_lists.read(np.uint32, max(1, lists.size))[:lists.size]
+    assert np.array_equal(g_off, offs) and np.array_equal(g_lists, lists)
+    kname = kernel_name(args, int(scene.info().max_depth))
+    workload = "soup-3M-bounce2-cull4096"
+    n_lights = int(scene.info().num_lights)
+    result = {
         "metric": "incoherent closest-hit Mrays/s (2 diffuse bounces) + tile light culling",
         "value": round(2 * n / ((ms1 + ms2) * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "soup-3M-bounce2-cull4096", "triangles": int(b.stats().num_triangles),
-                   "lights": 4096, "rays_per_bounce": n, "tiles": ntiles},
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms1 + ms2, 4), "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": workload, "triangles": int(b.stats().num_triangles),
+                   "lights": n_lights, "rays_per_bounce": n, "tiles": ntiles},
         "bounce1_mrays_s": round(n / (ms1 * 1e-3) / 1e6, 2), "bounce2_mrays_s": round(n / (ms2 * 1e-3) / 1e6, 2),
         "bounce1_hit_rate": round(float((hits1["tri_id"] != B.RTQ_NO_HIT).mean()), 4),
         "bounce2_hit_rate": round(float((hits2["tri_id"] != B.RTQ_NO_HIT).mean()), 4),
-        "cull_ms_incl_host_sync": round(cull_ms, 3), "tile_light_pairs": int(lists.size),
-        "dispatch": dispatch_cfg, "kernel": kernel_name(args, int(scene.info().max_depth)),
+        "roofline": {"bounce1": roofline(workload, f"{kname}@bounce1", ms1),
+                     "bounce2": roofline(workload, f"{kname}@bounce2", ms2)},
+        "cull": {"ms_per_call": round(cull_ms, 4), "tile_light_pairs": int(lists.size),
+                 "tile_light_tests_per_s": round(ntiles * n_lights / (cull_ms * 1e-3), 1),
+                 "kernels": "kCullMask + kScan + kCullEmit, stream-ordered scratch, no host synchronisation",
+                 "roofline": roofline(workload, "cull", cull_ms)},
+        "dispatch": dispatch_cfg, "kernel": kname,
         "host": {"threads": threads, "build_s": round(build_s, 1)},
-    }), flush=True)
+    }
+    if not args.no_cpu_baseline:
+        import oracle  # the CPU baseline leg is the only place bench.py touches the oracle
+
+        blob, ranges = b.fill_gpu_buffer()
+        sc = oracle.Scene(blob, ranges, b.positions())
+        parity = {}
+        for name, rays, hits, nid in (("bounce1", rays1, hits1, nid1), ("bounce2", rays2, hits2, nid2)):
+            o = sc.trace_closest(rays, threads=threads)
+            bad = (hits["tri_id"] != o["tri_id"]) | (nid != o["nid"])
+            for k in ("t", "u", "v"):
+                bad |= hits[k].view(np.uint32) != o[k].view(np.uint32)
+            parity[name] = {"rays_checked": int(rays.size), "mismatches": int(bad.sum())}
+        t0 = time.perf_counter()
+        o_off, o_lists = sc.cull_lights(f, W, Hh)
+        cull_cpu_s = time.perf_counter() - t0
+        parity["cull"] = {"tiles_checked": ntiles,
+                          "mismatches": int((not np.array_equal(o_off, offs)) or (not np.array_equal(o_lists, lists)))}
+        result["parity"] = parity
+        value, desc = timed_oracle(lambda r, th: sc.trace_closest(r, threads=th), rays1, threads, args.cpu_seconds,
+                                   "bounce-1 diffuse rays, closest hit, bitstack restatement of traverseBvh")
+        result["cpu_baseline"] = {"value": round(value, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                                  "sample": desc, "cpu_model": cpu_model(),
+                                  "cull": {"ms": round(cull_cpu_s * 1e3, 2), "threads": 1,
+                                           "sample": f"all {ntiles} tiles x {n_lights} lights, orc_cull_lights"}}
+    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

@@ -17,7 +17,13 @@ Parity pins (what checks each oracle function, see tests/test_oracle_pins.py):
   probe field tracing   tiled probe numbering inverted by the reference's getLightProbIndex; axis
                         rays from probes in a closed box hit the wall at the known distance
                         (planted mistake 6 fails)
-  light culling         closed forms (light on the view axis, light behind the camera, huge radius)
+  light culling         closed forms (light on the view axis, light behind the camera, huge radius);
+                        lights inside and outside the quarter-step widening band of a tile
+                        (planted mistake 7 fails)
+  camera rays           closed forms of the pinhole model; the reference storage row
+                        16*ceil(H/16) - y on a height that is not a multiple of 16 (planted mistake 8
+                        fails)
+  FP64 hits             closed forms on axis-aligned dyadic triangles
 """
 from __future__ import annotations
 
@@ -39,6 +45,7 @@ _SIGS = {
     "orc_collide_box": ([P, P, P, P, C.c_float], C.c_float),
     "orc_collide_triangle": ([P, P, P, P, P, C.c_float, P, P], C.c_float),
     "orc_intersect_f64": ([P, P, P, P, P, P, P, P], C.c_int),
+    "orc_hits_f64": ([P, C.c_size_t, P, P, U32, P, U32, P, P, P, P], C.c_int),
     "orc_trace_closest": ([P, C.c_size_t, P, P, U32, P, U32, P, P, P, P, P, P, P, C.c_int, C.c_int], C.c_int),
     "orc_trace_any": ([P, C.c_size_t, P, P, U32, P, U32, P, P, C.c_int, C.c_int], C.c_int),
     "orc_brute_closest": ([P, C.c_size_t, P, P, U32, P, U32, C.c_float, P, P, P], C.c_int),
@@ -137,6 +144,16 @@ class Scene:
         assert rc == 0
         return tri, t, near
 
+    def hits_f64(self, rays: np.ndarray, tri_id: np.ndarray):
+        """FP64 (t, u, v) of each ray against its recorded hit triangle; NaN for misses."""
+        n = rays.size
+        out = [np.zeros(n, np.float64) for _ in range(3)]
+        rc = lib().orc_hits_f64(_p(self.blob), self.blob.size, _p(self.ranges), _p(self.pos), self.nv,
+                                _p(np.ascontiguousarray(rays)), n, _p(np.ascontiguousarray(tri_id, np.uint32)),
+                                *[_p(o) for o in out])
+        assert rc == 0, "bad triangle id or blob"
+        return tuple(out)
+
     def shadow_masks(self, rays, hit_t, hit_nid, sun_dir, flags, threads: int = 8):
         n = rays.size
         masks = np.zeros(n, np.uint32)
@@ -189,23 +206,23 @@ def frustum_array(f) -> np.ndarray:
     return np.array(list(f.camera_pos) + list(f.corner00) + list(f.corner10) + list(f.corner01), np.float32)
 
 
-def cull_lights(lights: np.ndarray, frustum, width: int, height: int):
+def cull_lights(lights: np.ndarray, frustum, width: int, height: int, plant: int = 0):
     """lights: (n, 14) float32 PackedLight records. -> (offsets, lists)"""
     fr = frustum_array(frustum)
     ntiles = ((width + 15) // 16) * ((height + 15) // 16)
     offs = np.zeros(ntiles + 1, np.uint32)
     L = np.ascontiguousarray(lights, np.float32)
-    total = lib().orc_cull_lights(_p(L), L.shape[0], _p(fr), width, height, _p(offs), None, 0)
+    total = lib(plant).orc_cull_lights(_p(L), L.shape[0], _p(fr), width, height, _p(offs), None, 0)
     lists = np.zeros(max(1, total), np.uint32)
-    lib().orc_cull_lights(_p(L), L.shape[0], _p(fr), width, height, _p(offs), _p(lists), total)
+    lib(plant).orc_cull_lights(_p(L), L.shape[0], _p(fr), width, height, _p(offs), _p(lists), total)
     return offs, lists[:total]
 
 
-def camera_rays(frustum, width: int, height: int, reference_rows: bool = False) -> np.ndarray:
+def camera_rays(frustum, width: int, height: int, reference_rows: bool = False, plant: int = 0) -> np.ndarray:
     """Primary rays exactly as the camera pass generates them (rtq_ray records, y*W + x)."""
     fr = frustum_array(frustum)
     out = np.zeros(width * height * 8, np.float32)
-    lib().orc_camera_rays(_p(fr), width, height, int(reference_rows), _p(out))
+    lib(plant).orc_camera_rays(_p(fr), width, height, int(reference_rows), _p(out))
     return out.view(np.dtype([("org", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("pad", "<u4")]))
 
 

@@ -36,7 +36,8 @@ typedef uint32_t u32;
 /* Planted mistakes, used only by the tests that check the pins catch them:
  * 1 = tie-break to the far child (d1 <= d0), 2 = no t shrink on acceptance, 3 = u clamp dropped,
  * 4 = probe cell without the half-cell shift, 5 = probe cell not capped at resolution - 2,
- * 6 = probe position without the half-step offset. */
+ * 6 = probe position without the half-step offset, 7 = tile corners not widened by a quarter step,
+ * 8 = reference storage row off by one (16 * ceil(H / 16) - y - 1). */
 static const int kPlant = ORC_PLANT;
 #else
 static const int kPlant = 0;
@@ -527,6 +528,27 @@ typedef struct {
   u32 begin, end;
 } orc_shadow_job;
 
+/* FP64 intersection of each ray with its recorded closest-hit triangle, for error statements of the
+ * fp32 path: for ray i with tri_id[i] != 0xFFFFFFFF, orc_intersect_f64 of that triangle (vertices
+ * through the blob's Triangle record, loadTriangleVertices); NaN for misses. Returns -1 on a bad blob,
+ * a triangle id beyond the triangle range or a vertex beyond the positions. */
+int orc_hits_f64(const uint8_t* blob, size_t bytes, const u32* ranges, const float* pos, u32 nv, const float* rays,
+                 u32 n, const u32* tri_id, double* t, double* u, double* v) {
+  orc_view view;
+  if (view_init(&view, blob, bytes, ranges, pos, nv)) { view_free(&view); return -1; }
+  int rc = 0;
+  for (u32 i = 0; i < n; ++i) {
+    t[i] = u[i] = v[i] = NAN;
+    if (tri_id[i] == 0xFFFFFFFFu) continue;
+    float p[3][3];
+    if (tri_id[i] >= view.num_tris || load_vertices(&view, view.tris + 3 * (size_t)tri_id[i], p)) { rc = -1; break; }
+    const float* r = rays + 8 * (size_t)i;
+    if (!orc_intersect_f64(r, r + 4, p[0], p[1], p[2], t + i, u + i, v + i)) t[i] = u[i] = v[i] = NAN;
+  }
+  view_free(&view);
+  return rc;
+}
+
 static int any_hit(const orc_view* v, const float* o, const float* d, float tmax) {
   orc_hit h;
   memset(&h, 0, sizeof(h));
@@ -625,7 +647,7 @@ void orc_camera_rays(const float* frustum, u32 width, u32 height, int reference_
     for (u32 x = 0; x < width; ++x) {
       float* r = out + 8 * ((size_t)y * width + x);
       u32 gy = y;
-      if (reference_rows) gy = 16u * ((height + 15u) / 16u) - y;
+      if (reference_rows) gy = 16u * ((height + 15u) / 16u) - y - (kPlant == 8 ? 1u : 0u);
       float d[3];
       for (int k = 0; k < 3; ++k) {
         float sw = (c10[k] - c00[k]) * invW;
@@ -876,7 +898,7 @@ void orc_tile_planes(const float* frustum, u32 width, u32 height, u32 tx, u32 ty
     tr[k] = (c00[k] + sx[k] * fx1) + sy[k] * fy1;
   }
   for (int k = 0; k < 3; ++k) {
-    float dx = sx[k] * 0.25f, dy = sy[k] * 0.25f;
+    float dx = sx[k] * (kPlant == 7 ? 0.0f : 0.25f), dy = sy[k] * (kPlant == 7 ? 0.0f : 0.25f);
     bl[k] = (bl[k] - dx) - dy;
     br[k] = (br[k] + dx) - dy;
     tl[k] = (tl[k] - dx) + dy;

@@ -35,3 +35,12 @@ def assert_closest_equal(g, o, label=""):
         ob = np.asarray(o[k], np.float32).view(np.uint32)
         bad = np.flatnonzero(gb != ob)
         assert bad.size == 0, f"{label} {k} bits differ on {bad.size} rays, first {bad[:5]}"
+
+
+def tile_order(width: int, height: int, t: int = 8) -> np.ndarray:
+    """Pixel indices y*width + x listed in t x t tiles (tiles row-major, pixels row-major inside a
+    tile), the ray order of the benchmark's primary buffer."""
+    assert width % t == 0 and height % t == 0
+    y, x = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
+    idx = (y * width + x).reshape(height // t, t, width // t, t).transpose(0, 2, 1, 3)
+    return np.ascontiguousarray(idx.reshape(-1))

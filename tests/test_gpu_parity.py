@@ -437,14 +437,47 @@ def test_large_dispatch_16m_rays():
     assert (grid["tri_id"] != B.RTQ_NO_HIT).mean() > 0.1
 
 
-def test_full_size_spheres_subsample():
-    _print("case: full 1M-tri sphere scene, 1920x1080 rays, oracle on a 65536-ray sample")
+def test_full_size_spheres_every_ray():
+    """The headline workload exactly as the benchmark times it: 1M-triangle spheres, 1920x1080 jittered
+    primary rays laid out in 8x8 pixel tiles (whole waves take the octant form of the box test), the
+    default grid dispatch; every ray against the oracle, then the full-frame shadow pass (the point
+    light of every leaf list, with and without the sun) against the oracle's masks."""
+    _print("case: full 1M-tri sphere scene, 1920x1080 tile-ordered rays, every ray + full-frame shadow masks")
     s = G.scene(2, 1.0)
     rays = H.frustum_rays(s["frustum"], 1920, 1080, seed=2)
+    rays = np.ascontiguousarray(rays[G.tile_order(1920, 1080)])
     g = s["gpu"].trace_closest(rays)
-    idx = np.random.default_rng(2).choice(rays.size, 65536, replace=False)
-    o = s["orc"].trace_closest(rays[idx], threads=16)
-    sub = {k: g[k][idx] for k in ("tri_id", "nid", "t", "u", "v")}
-    G.assert_closest_equal(sub, o, "full spheres sample")
-    hit = np.flatnonzero(g["tri_id"] != B.RTQ_NO_HIT)
-    assert hit.size > 0.2 * rays.size
+    o = s["orc"].trace_closest(rays, threads=16)
+    G.assert_closest_equal(g, o, "full spheres, every ray")
+    hit = o["tri_id"] != B.RTQ_NO_HIT
+    assert hit.mean() > 0.2
+    sun = np.array([0.3, 0.2, -1.0], np.float32)
+    sun /= np.linalg.norm(sun)
+    for flags in (0, B.RTQ_SHADOW_SUN):
+        gm = s["gpu"].shadow(rays, g["hits"], g["nid"], sun, flags)
+        om = s["orc"].shadow_masks(rays, o["t"], o["nid"], sun, flags, threads=16)
+        assert np.array_equal(gm, om), f"flags {flags}: {np.count_nonzero(gm != om)} masks differ"
+        assert ((om[hit] >> 1) & 1).any() and (((om[hit] >> 1) & 1) == 0).any()
+
+
+def test_cache_loaded_scene_traces_identically(tmp_path):
+    """A scene saved to the on-disk cache and loaded back (blob, ranges and positions from the file,
+    not from the builder) uploads and traces bit for bit like the built one and like the oracle."""
+    _print("case: on-disk cache round trip -> upload -> closest, any and shadow, spheres cut")
+    s = G.scene(2, 1 / 16)
+    path = str(tmp_path / "spheres.rtqbvh")
+    s["builder"].save_cache(path)
+    loaded = H.load_cache(path, s["builder"].input_hash())
+    assert loaded is not None
+    blob, ranges, pos, _ = loaded
+    cached = D.Scene(G.dev(), blob, ranges, pos)
+    f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 640 / 360)
+    rays = H.frustum_rays(f, 640, 360, seed=51)
+    g = cached.trace_closest(rays)
+    G.assert_closest_equal(g, s["orc"].trace_closest(rays, threads=16), "cache-loaded vs oracle")
+    G.assert_closest_equal(g, s["gpu"].trace_closest(rays), "cache-loaded vs built")
+    m = cached.shadow(rays, g["hits"], g["nid"], (0, 0, -1), B.RTQ_SHADOW_SUN)
+    assert np.array_equal(m, s["orc"].shadow_masks(rays, g["t"], g["nid"], np.float32([0, 0, -1]), B.RTQ_SHADOW_SUN,
+                                                   threads=16))
+    rays["tmax"] = np.float32(35.0)
+    assert np.array_equal(cached.trace_any(rays), s["orc"].trace_any(rays, threads=16)[0])

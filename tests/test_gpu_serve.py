@@ -5,6 +5,7 @@ import threading
 import numpy as np
 import pytest
 
+import oracle
 from tests import gpu_util as G
 from tim_raytracing_amd import bindings as B
 from tim_raytracing_amd import device as D
@@ -43,14 +44,18 @@ def test_serve_mix_equals_alone():
     builders = _scenes(8, 7)
     scenes = [D.Scene.from_builder(dev, b) for b in builders]
     reqs = _requests(builders, 40, 8)
-    # alone: one dispatch per request
+    # alone: one dispatch per request, each checked against the oracle on every ray
+    orc = [oracle.Scene(*b.fill_gpu_buffer(), b.positions()) for b in builders]
     alone = []
     for si, flags, rays in reqs:
         if flags == B.RTQ_TRACE_CLOSEST:
             g = scenes[si].trace_closest(rays)
+            G.assert_closest_equal(g, orc[si].trace_closest(rays, threads=16), "served request alone vs oracle")
             alone.append((g["hits"].view(np.uint32).copy(), g["nid"]))
         else:
-            alone.append((scenes[si].trace_any(rays), None))
+            occ = scenes[si].trace_any(rays)
+            assert np.array_equal(occ, orc[si].trace_any(rays, threads=16)[0])
+            alone.append((occ, None))
     # mixed: a budget that holds only part of the stream at once, 4 submitting threads; chunked and
     # lane-refilling batch kernels
     for params in (None, B.BatchParams(B.RTQ_BATCH_REFILL, 8, 64, 0, 64)):

@@ -569,3 +569,79 @@ def test_camera_model_closed_form():
     # one pixel step: the corner grid, not the pixel centres
     step = unit(c00 + w_vec / W - cam)
     assert np.allclose(d[0, 1], step, atol=1e-6)
+
+
+# ---------------------------------------------------------------- tile widening (cullFrustum.glsl:75-80)
+@pytest.mark.parametrize("plant", [0, 7])
+def test_culling_quarter_step_widening(plant):
+    """A 90-degree frustum on exactly representable corners, oriented as the camera builds it (looking
+    along +x with z up, so x pixels run toward -y), 32x32 pixels = 2x2 tiles: at x = 1 tile (tx, ty)
+    spans y in [-tx, 1 - tx] and z in [ty - 1, ty], and the shader widens its corners by a quarter tile
+    step on every side, so a light is listed when its y/x lies in [-tx - 0.25, 1.25 - tx] and its z/x in
+    [ty - 1.25, ty + 0.25] (tiny radius, every ratio 0.05 or more from a bound). Lights in the widening
+    band of a tile are listed by it too; dropping the widening (planted mistake 7) or flipping its sign
+    loses them."""
+    fr = np.array([0, 0, 0, 1, 1, -1, 1, -1, -1, 1, 1, 1], F32)
+    ry = [-1.3, -1.2, -0.6, -0.2, 0.1, 0.3, 0.6, 1.2, 1.3]
+    rz = [-1.2, -0.1, 0.2, 0.6, 1.3]
+    pos_r = [((10.0, 10.0 * a, 10.0 * b), 0.001) for a in ry for b in rz]
+    offs, lists = oracle.cull_lights(_light_records(pos_r), fr, 32, 32, plant=plant)
+    ok = True
+    for ty in range(2):
+        for tx in range(2):
+            want = [i for i, ((x, y, z), _) in enumerate(pos_r)
+                    if -tx - 0.25 <= y / x <= 1.25 - tx and ty - 1.25 <= z / x <= ty + 0.25]
+            t = ty * 2 + tx
+            ok &= lists[offs[t]:offs[t + 1]].tolist() == want
+    assert ok == (plant == 0)
+
+
+# ---------------------------------------------------------------- storage row (cameraPass.comp:46)
+@pytest.mark.parametrize("plant", [0, 8])
+def test_camera_reference_storage_row_closed_form(plant):
+    """With the reference's row mapping, stored row y >= 1 holds the ray generated for row
+    gy = 16 * ceil(H / 16) - y (48 - y for H = 36, so some rays point past the top corner) and row 0 is
+    never written. On corners (1, -1, -1), (1, 1, -1), (1, -1, 1) from the origin the direction of pixel
+    (x, gy) is proportional to (1, -1 + 2x/W, -1 + 2gy/H); a row off by one (planted mistake 8) moves
+    z/x by 2/36."""
+    W, Hh = 32, 36
+    fr = np.array([0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 1], F32)
+    rays = oracle.camera_rays(fr, W, Hh, reference_rows=True, plant=plant)
+    d = rays["dir"].astype(np.float64).reshape(Hh, W, 3)
+    assert np.all(rays["tmax"].reshape(Hh, W)[0] < 0)  # row 0: not written by the pass
+    y, x = np.meshgrid(np.arange(1, Hh), np.arange(W), indexing="ij")
+    gy = 48 - y
+    ok = np.allclose(d[1:, :, 1] / d[1:, :, 0], -1 + 2 * x / W, atol=1e-5) and \
+        np.allclose(d[1:, :, 2] / d[1:, :, 0], -1 + 2 * gy / Hh, atol=1e-5)
+    assert ok == (plant == 0)
+    # without the mapping the stored row is the generating row
+    plain = oracle.camera_rays(fr, W, Hh, reference_rows=False, plant=plant)["dir"].astype(np.float64)
+    plain = plain.reshape(Hh, W, 3)
+    yy = np.arange(Hh)[:, None]
+    assert np.allclose(plain[:, :, 2] / plain[:, :, 0], -1 + 2 * yy / Hh, atol=1e-5)
+
+
+# ---------------------------------------------------------------- FP64 hit evaluation (error statements)
+def test_hits_f64_closed_form():
+    """orc_hits_f64 on axis-aligned dyadic triangles, where t, u and v are exact: the ray hits triangle
+    `tri_id` (not another one), u and v are the weights of the second and third vertex, misses are NaN."""
+    b = H.BVHBuilder()
+    off = b.add_vertices(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 5], [2, 0, 5], [0, 2, 5]], np.float32))
+    b.add_triangle_list(off, [0, 1, 2])
+    b.add_triangle_list(off, [3, 4, 5])
+    b.build(1)
+    blob, r = b.fill_gpu_buffer()
+    sc = oracle.Scene(blob, r, b.positions())
+    rays = np.zeros(3, B.RAY_DTYPE)
+    rays["org"] = [[0.25, 0.5, 2], [0.5, 0.25, 10], [0.1, 0.1, 3]]
+    rays["dir"] = [[0, 0, -1], [0, 0, -1], [0, 0, -1]]
+    rays["tmax"] = 100
+    # which triangle id the builder gave each list: the blob's triangle records in insertion order
+    t, u, v = sc.hits_f64(rays, np.array([0, 1, 0xFFFFFFFF], np.uint32))
+    assert (t[0], u[0], v[0]) == (2.0, 0.25, 0.5)
+    assert (t[1], u[1], v[1]) == (5.0, 0.25, 0.125)
+    assert np.isnan(t[2]) and np.isnan(u[2]) and np.isnan(v[2])
+    # the fp32 closest hit picks the same triangles and agrees within fp32 rounding
+    o = sc.trace_closest(rays[:2], threads=1)
+    assert o["tri_id"].tolist() == [0, 1]
+    assert np.allclose(o["t"] + np.float32(1e-5), t[:2], rtol=1e-6)

@@ -450,6 +450,14 @@ class Scene:
         self.shadow_masks(d_r.ptr, d_h.ptr, d_n.ptr, n, sun_dir, flags, d_m.ptr)
         return d_m.read(np.uint32, n)
 
+    def cull_lights_async(self, frustum: B.Frustum, width: int, height: int, d_offsets: int, d_lists: int,
+                          capacity: int, stream=None):
+        """rtq_cull_lights without h_total: asynchronous on `stream`, the total lands in
+        d_offsets[num_tiles]."""
+        B.check_device("rtq_cull_lights",
+                       self.dev.lib.rtq_cull_lights(self.handle, C.byref(frustum), width, height, d_offsets, d_lists,
+                                                    capacity, None, stream))
+
     def cull_lights(self, frustum: B.Frustum, width: int, height: int, capacity: int | None = None):
         ntiles = ((width + 15) // 16) * ((height + 15) // 16)
         d_off = self.dev.alloc(4 * (ntiles + 1))

@@ -262,6 +262,45 @@ def check_against_oracle(sc, rays, g_hits, g_nid, g_masks, threads: int) -> tupl
     return parity, err
 
 
+def oracle_samples(groups, threads: int, target_s: float):
+    """groups: per scene (oracle scene, [(rays, closest?, gpu result, gpu leaf ids or None), ...]), each
+    ray set a sample of a batch item or request. Checks every sampled ray against the oracle (closest:
+    triangle id, leaf id and the bits of t, u, v; any: the flag), then times the oracle's scalar
+    traversal over the same samples for about target_s. -> (parity, Mrays/s, rays per pass)"""
+    checked = bad = 0
+    for sc, sets in groups:
+        for rays, closest, got, nid in sets:
+            checked += rays.size
+            if closest:
+                o = sc.trace_closest(rays, threads=threads)
+                m = got["tri_id"] != o["tri_id"]
+                if nid is not None:
+                    m |= nid != o["nid"]
+                for k in ("t", "u", "v"):
+                    m |= got[k].view(np.uint32) != o[k].view(np.uint32)
+                bad += int(m.sum())
+            else:
+                bad += int((got != sc.trace_any(rays, threads=threads)[0]).sum())
+    # timing: one oracle call per scene over its concatenated samples
+    merged = [(sc, np.concatenate([r for r, c, _, _ in sets if c]) if any(c for _, c, _, _ in sets) else None,
+               np.concatenate([r for r, c, _, _ in sets if not c]) if any(not c for _, c, _, _ in sets) else None)
+              for sc, sets in groups]
+    per_pass = sum((a.size if a is not None else 0) + (b.size if b is not None else 0) for _, a, b in merged)
+    passes = 0
+    t0 = time.perf_counter()
+    while True:
+        for sc, a, b in merged:
+            if a is not None:
+                sc.trace_closest(a, threads=threads)
+            if b is not None:
+                sc.trace_any(b, threads=threads)
+        passes += 1
+        if time.perf_counter() - t0 >= target_s or passes >= 100:
+            break
+    dt = time.perf_counter() - t0
+    return {"rays_checked": checked, "mismatches": bad}, passes * per_pass / dt / 1e6, per_pass, passes, dt
+
+
 def cpu_config1(threads: int) -> dict:
     """The CPU method on the closed box (16 384 triangles): the builder restatement's wall time and the
     oracle's scalar closest-hit traversal of 256x256 jittered primary rays, on 1 thread and on `threads`."""
@@ -539,72 +578,30 @@ def bench_copy(args):
                       "pinned_link_gb_s": {"h2d": h2d, "d2h": d2h, "bytes": hbytes}}), flush=True)
 
 
-def _loguniform(rng, lo, hi, n):
-    return np.exp(rng.uniform(np.log(lo), np.log(hi), n))
-
-
-def _scene_pool(sizes, distinct, threads, cache_dir=None):
-    """One sphere-soup scene per distinct size (UV spheres of 256 triangles, seeds 1000+i): a list of
-    (builder, blob, ranges, positions). With cache_dir, built scenes are stored there under the hash
-    of their inputs and later runs load them instead of building."""
-    from tim_raytracing_amd import bindings as B
-    from tim_raytracing_amd import host as H
-
-    pool = []
-    for i, tris in enumerate(sizes[:distinct]):
-        b = H.BVHBuilder()
-        b.generate(B.RTQ_SCENE_SPHERES, 1000 + i, max(1, int(round(tris / 256))))
-        path = None
-        if cache_dir:
-            os.makedirs(cache_dir, exist_ok=True)
-            key = b.input_hash()
-            path = os.path.join(cache_dir, f"{key:016x}.rtqbvh")
-            hit = H.load_cache(path, key)
-            if hit is not None:
-                pool.append((b, hit[0], hit[1], hit[2]))
-                continue
-        b.build(threads)
-        if path:
-            b.save_cache(path)
-        blob, ranges = b.fill_gpu_buffer()
-        pool.append((b, blob, ranges, b.positions()))
-        if (i + 1) % 16 == 0:
-            print(f"built {i + 1} of {min(distinct, len(sizes))} scenes", file=sys.stderr, flush=True)
-    return pool
-
-
-def _primary_rays(n, seed):
-    from tim_raytracing_amd import host as H
-
-    side = int(np.ceil(np.sqrt(n)))
-    f = H.camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), H.REFERENCE_FOV, 1.0)
-    return np.ascontiguousarray(H.frustum_rays(f, side, side, seed=seed)[:n])
-
-
 def bench_batch(args):
     """Batched form: 256 scenes of LogU[1K, 2M] triangles, LogU[4K, 1M] rays each (half primary,
     half bounce), one persistent launch (autotuned, hipGraph replay) vs one launch per item."""
     from tim_raytracing_amd import bindings as B
     from tim_raytracing_amd import device as D
 
+    from tim_raytracing_amd import host as H
+
     threads = host_threads(args.threads)
-    rng = np.random.default_rng(4)
     slots = 256
-    sizes = _loguniform(rng, 1000, 2e6 * args.scale, slots)
+    builders, ray_counts = H.batch_workload(args.scale, slots)
     distinct = args.distinct or slots
     t0 = time.perf_counter()
-    pool = _scene_pool(sizes, distinct, threads, args.cache_dir)
-    builders = [p[0] for p in pool]
+    pool = H.build_many(builders[:distinct], threads, args.cache_dir, progress=lambda m: print(m, file=sys.stderr,
+                                                                                               flush=True))
     build_s = time.perf_counter() - t0
     dev = D.Device(0)
-    scenes = [D.Scene(dev, blob, r, pos) for _, blob, r, pos in pool]
-    ray_counts = _loguniform(rng, 4096, 1 << 20, slots).astype(np.int64)
+    scenes = [D.Scene(dev, blob, r, pos) for blob, r, pos in pool]
     items, keep = [], []
     total = 0
     for i in range(slots):
         si = i % distinct
         n = int(ray_counts[i])
-        rays = _primary_rays(n, 10 + i) if i % 2 == 0 else builders[si].surface_rays(seed=10 + i, count=n)
+        rays = H.batch_item_rays(builders[si], i, n)
         d_r = dev.upload(rays)
         d_o = dev.alloc(16 * n)
         d_n = dev.alloc(4 * n)
@@ -632,21 +629,46 @@ def bench_batch(args):
     batch.set_params(use_graph=1)
     batched_ms = timed(lambda: batch.launch(sptr))
     p = batch.params()
-    print(json.dumps({
+    kname = f"{'kBatchRefill' if p.chunk_rays == B.RTQ_BATCH_REFILL else 'kBatchPersistent'}<{p.stack_depth}>"
+    result = {
         "metric": "batched closest-hit Mrays/s vs one launch per item",
         "value": round(total / (batched_ms * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(batched_ms, 4),
         "higher_is_better": True, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batch-256-scenes", "scene_slots": slots, "distinct_bvhs": distinct,
                    "triangles_per_slot": "LogU[1e3, %g]" % (2e6 * args.scale),
-                   "distinct_triangles": int(sum(r.triangles.size // 12 for _, _, r, _ in pool)),
+                   "distinct_triangles": int(sum(r.triangles.size // 12 for _, r, _ in pool)),
                    "rays": int(total), "rays_per_item": "LogU[4096, 1048576]", "mix": "50% primary, 50% bounce"},
         "plain_mrays_s": round(total / (plain_ms * 1e-3) / 1e6, 2), "plain_ms": round(plain_ms, 4),
         "speedup_vs_plain": round(plain_ms / batched_ms, 3),
         "autotuned": {"chunk_rays": p.chunk_rays, "blocks_per_cu": p.blocks_per_cu, "stack_depth": p.stack_depth,
                       "use_graph": p.use_graph, "leaf_wait": p.leaf_wait, "tuning_ms": round(tuned_ms, 4)},
+        "roofline": roofline("batch-256-scenes" if distinct == slots else f"batch-256-scenes-{distinct}-bvhs", kname,
+                             batched_ms),
         "host": {"threads": threads, "build_s": round(build_s, 1)},
-    }), flush=True)
+    }
+    if not args.no_cpu_baseline:
+        import oracle  # the CPU baseline leg is the only place bench.py touches the oracle
+
+        # the graph-replayed launch's results of every item, sampled (4096 strided rays per item)
+        groups = {}
+        for i in range(slots):
+            si = i % distinct
+            d_r, d_o, d_n = keep[i]
+            n = int(ray_counts[i])
+            idx = np.arange(0, n, max(1, n // 4096))[:4096]
+            rays = H.batch_item_rays(builders[si], i, n)[idx]
+            hits = d_o.read(B.HIT_DTYPE, n)[idx]
+            nid = d_n.read(np.uint32, n)[idx]
+            groups.setdefault(si, []).append((np.ascontiguousarray(rays), True, hits, nid))
+        scs = [(oracle.Scene(*pool[si]), sets) for si, sets in groups.items()]
+        parity, rate, per_pass, passes, dt = oracle_samples(scs, threads, args.cpu_seconds)
+        result["parity"] = dict(parity, checker="oracle traverseBvh on 4096 strided rays of every item")
+        result["cpu_baseline"] = {"value": round(rate, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                                  "sample": f"{per_pass} rays (4096 strided rays of each of the {slots} items) x "
+                                            f"{passes} passes, closest hit, bitstack restatement of traverseBvh, "
+                                            f"{threads} threads, {dt:.1f} s", "cpu_model": cpu_model()}
+    print(json.dumps(result), flush=True)
 
 
 def bench_serve(args):
@@ -660,18 +682,20 @@ def bench_serve(args):
     from tim_raytracing_amd import bindings as B
     from tim_raytracing_amd import device as D
 
+    from tim_raytracing_amd import host as H
+
     threads = host_threads(args.threads)
-    rng = np.random.default_rng(5)
     slots = 64
-    sizes = _loguniform(rng, 1e4, 3e6 * args.scale, slots)
+    nreq = args.requests
+    builders, rng, req_sizes, req_scene, req_any = H.serve_workload(args.scale, slots, nreq)
     distinct = args.distinct or slots
     t0 = time.perf_counter()
-    pool = _scene_pool(sizes, distinct, threads, args.cache_dir)
-    builders = [p[0] for p in pool]
+    pool = H.build_many(builders[:distinct], threads, args.cache_dir, progress=lambda m: print(m, file=sys.stderr,
+                                                                                               flush=True))
     build_s = time.perf_counter() - t0
     dev = D.Device(0)
     # every slot is its own resident copy in HBM, also where slots share a distinct BVH
-    scenes = [D.Scene(dev, *pool[i % distinct][1:]) for i in range(slots)]
+    scenes = [D.Scene(dev, *pool[i % distinct]) for i in range(slots)]
     max_depth = max(int(s.info().max_depth) for s in scenes)
     pool_rays = 1 << 21
     pinned_pools = []
@@ -680,10 +704,6 @@ def bench_serve(args):
         hb.view(B.RAY_DTYPE, pool_rays)[:] = builders[i].surface_rays(seed=50 + i, count=pool_rays)
         pinned_pools.append(hb)
     pools = [hb.view(B.RAY_DTYPE, pool_rays) for hb in pinned_pools]
-    nreq = args.requests
-    req_sizes = _loguniform(rng, 1000, 2e6, nreq).astype(np.int64)
-    req_scene = rng.integers(0, slots, nreq)
-    req_any = rng.random(nreq) < 0.5
     req_off = [int(rng.integers(0, pool_rays - n + 1)) for n in req_sizes]
     out_off = np.concatenate([[0], np.cumsum(np.where(req_any, 4, 16) * req_sizes)]).astype(np.int64)
     out_arena = dev.host_alloc(int(out_off[-1]))
@@ -735,8 +755,35 @@ def bench_serve(args):
     rate = 1.5 * plain_cap["mrays_s"] * 1e6 / float(req_sizes.mean())
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     served = run(False, arrivals)
+    checks = {}
+    if not args.no_cpu_baseline:
+        import oracle  # the CPU baseline leg is the only place bench.py touches the oracle
+
+        # the at-load served results of every request, sampled (4096 strided rays per request)
+        groups = {}
+        for k in range(nreq):
+            si = int(req_scene[k])
+            n = int(req_sizes[k])
+            idx = np.arange(0, n, max(1, n // 4096))[:4096]
+            rays = np.ascontiguousarray(pools[si % distinct][req_off[k]:req_off[k] + n][idx])
+            if req_any[k]:
+                got = out_arena.view(np.uint32, n, int(out_off[k]))[idx].copy()
+                groups.setdefault(si % distinct, []).append((rays, False, got, None))
+            else:
+                got = out_arena.view(B.HIT_DTYPE, n, int(out_off[k]))[idx].copy()
+                groups.setdefault(si % distinct, []).append((rays, True, got, None))
+        scs = [(oracle.Scene(*pool[si]), sets) for si, sets in groups.items()]
+        parity, rate, per_pass, passes, dt = oracle_samples(scs, threads, args.cpu_seconds)
+        checks["parity"] = dict(parity, checker="oracle traverseBvh / traverseBvhFast on 4096 strided rays of "
+                                                "every served request (at-load run)")
+        checks["cpu_baseline"] = {"value": round(rate, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                                  "sample": f"{per_pass} rays (4096 strided rays of each of the {nreq} requests, "
+                                            f"half closest, half any hit) x {passes} passes, bitstack restatement "
+                                            f"of traverseBvh / traverseBvhFast, {threads} threads, {dt:.1f} s",
+                                  "cpu_model": cpu_model()}
     plain = run(True, arrivals)
-    print(json.dumps({
+    kname = f"{'kBatchRefill' if params.chunk_rays == B.RTQ_BATCH_REFILL else 'kBatchPersistent'}<{params.stack_depth}>"
+    result = {
         "metric": "served aggregate Mrays/s and p50/p99 latency vs one launch per request",
         "value": served["mrays_s"], "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
         "data": "synthetic",
@@ -753,8 +800,14 @@ def bench_serve(args):
         "speedup_vs_plain": round(served["mrays_s"] / plain["mrays_s"], 3) if plain["mrays_s"] else None,
         "capacity": {"served": served_cap, "plain": plain_cap,
                      "speedup_vs_plain": round(served_cap["mrays_s"] / plain_cap["mrays_s"], 3)},
+        # over the saturated served run's makespan: the VALU instructions of all its launches (committed
+        # counters of one such run) per second of wall time, a lower bound of the kernels' own issue rate
+        "roofline": roofline("serve-64-scenes-poisson", f"{kname}@saturated-stream",
+                             served_cap["makespan_s"] * 1e3),
         "host": {"threads": threads, "build_s": round(build_s, 1)},
-    }), flush=True)
+    }
+    result.update(checks)
+    print(json.dumps(result), flush=True)
 
 
 def bench_incoherent(args):

@@ -44,3 +44,31 @@ def tile_order(width: int, height: int, t: int = 8) -> np.ndarray:
     y, x = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
     idx = (y * width + x).reshape(height // t, t, width // t, t).transpose(0, 2, 1, 3)
     return np.ascontiguousarray(idx.reshape(-1))
+
+
+@functools.lru_cache(maxsize=None)
+def batch_pool():
+    """The batched form's full workload (256 scenes of LogU[1K, 2M] triangles), built once per process
+    on 16 host threads: (builders, rays per item, [(blob, ranges, positions)] per scene)."""
+    import time
+
+    builders, counts = H.batch_workload()
+    t0 = time.time()
+    pool = H.build_many(builders, threads=16, progress=lambda m: print(m, flush=True))
+    print(f"built {len(pool)} scenes, {sum(b.stats().num_triangles for b in builders)} triangles, in "
+          f"{time.time() - t0:.0f} s", flush=True)
+    return builders, counts, pool
+
+
+@functools.lru_cache(maxsize=None)
+def serve_pool(requests: int):
+    """The served form's full workload (64 resident scenes of LogU[10K, 3M] triangles) and a stream of
+    `requests` requests, built once per process on 16 host threads."""
+    import time
+
+    builders, rng, sizes, scenes, any_hit = H.serve_workload(requests=requests)
+    t0 = time.time()
+    pool = H.build_many(builders, threads=16, progress=lambda m: print(m, flush=True))
+    print(f"built {len(pool)} scenes, {sum(b.stats().num_triangles for b in builders)} triangles, in "
+          f"{time.time() - t0:.0f} s", flush=True)
+    return builders, rng, sizes, scenes, any_hit, pool

@@ -121,3 +121,56 @@ def test_batch_rejects_bad_items():
     b.launch()
     with pytest.raises(B.RtqError):
         b.set_params(chunk_rays=100)
+
+
+def test_batch_256_scenes_full_size():
+    """The batched form at its stated size: 256 independently built scenes of LogU[1K, 2M] triangles,
+    LogU[4K, 1M] rays per item (primary rays for even items, diffuse surface rays for odd ones), one
+    autotuned, graph-replayed persistent launch. Every ray of every replay equals one plain launch per
+    item, and the oracle agrees on 4096 strided rays of every item."""
+    print("case: batch of 256 independent scenes (LogU[1K, 2M] tris), LogU[4K, 1M] rays per item, "
+          "autotuned graph launch vs plain vs oracle samples", flush=True)
+    builders, counts, pool = G.batch_pool()
+    dev = G.dev()
+    scenes = [D.Scene(dev, *p) for p in pool]
+    items, bufs, host_rays = [], [], []
+    for i in range(len(scenes)):
+        n = int(counts[i])
+        rays = H.batch_item_rays(builders[i], i, n)
+        d_r, d_o, d_n = dev.upload(rays), dev.alloc(16 * n), dev.alloc(4 * n)
+        bufs.append((d_r, d_o, d_n))
+        host_rays.append(rays)
+        items.append((i, B.RTQ_TRACE_CLOSEST, d_r.ptr, n, d_o.ptr, d_n.ptr))
+    total = int(counts.sum())
+    print(f"{total} rays in {len(items)} items", flush=True)
+    batch = D.Batch(dev, scenes, items)
+    stream = dev.stream()
+    sptr = stream.ptr
+
+    def read_all():
+        return [(d_o.read(B.HIT_DTYPE, r.size).view(np.uint32).copy(), d_n.read(np.uint32, r.size))
+                for (d_r, d_o, d_n), r in zip(bufs, host_rays)]
+
+    batch.launch_plain(sptr)
+    stream.synchronize()
+    plain = read_all()
+    best, ms = batch.autotune(reps=1, stream=sptr)
+    print(f"autotuned: chunk {best.chunk_rays}, {best.blocks_per_cu} blocks/CU, stack {best.stack_depth}, "
+          f"wait {best.leaf_wait}: {ms:.2f} ms per launch ({total / ms / 1e3:.0f} Mrays/s)", flush=True)
+    batch.set_params(use_graph=1)
+    for replay in range(2):
+        for d_r, d_o, d_n in bufs:
+            d_o.write(np.zeros(d_o.nbytes, np.uint8))
+            d_n.write(np.zeros(d_n.nbytes, np.uint8))
+        batch.launch(sptr)
+        stream.synchronize()
+        for k, (a, b) in enumerate(zip(plain, read_all())):
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), f"replay {replay} item {k}"
+    for i, ((h, nid), rays) in enumerate(zip(plain, host_rays)):
+        idx = np.arange(0, rays.size, max(1, rays.size // 4096))[:4096]
+        o = oracle.Scene(*pool[i]).trace_closest(np.ascontiguousarray(rays[idx]), threads=16)
+        hv = h.reshape(-1, 4)[idx]
+        assert np.array_equal(hv[:, 0], o["tri_id"]), f"item {i} tri_id"
+        assert np.array_equal(hv[:, 1], o["t"].view(np.uint32)), f"item {i} t"
+        assert np.array_equal(hv[:, 2], o["u"].view(np.uint32)) and np.array_equal(hv[:, 3], o["v"].view(np.uint32))
+        assert np.array_equal(nid[idx], o["nid"]), f"item {i} nid"

@@ -1,6 +1,7 @@
 """Served form: results identical alone and in the mix, admission under the HBM budget,
 queue-full backpressure and cancellation."""
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -154,3 +155,84 @@ def test_serve_requests_outlive_server():
         assert h.cancel() != B.RTQ_OK  # finished: nothing left to cancel
         h.release()
     assert done >= 1
+
+
+def test_serve_64_scenes_poisson_full_size():
+    """The served form at its stated size: 64 resident scenes of LogU[10K, 3M] triangles and 240
+    Poisson-timed requests of LogU[1K, 2M] rays (half closest, half any hit) submitted by 4 client
+    threads under a 2 GiB budget. Every ray of every served request equals the request traced alone (one
+    plain dispatch), and the oracle agrees on 4096 strided rays of every request."""
+    print("case: 64 resident scenes (LogU[10K, 3M] tris), 240 Poisson requests of LogU[1K, 2M] rays from "
+          "4 threads, served vs alone vs oracle samples", flush=True)
+    nreq = 240
+    builders, rng, sizes, req_scene, req_any, pool = G.serve_pool(nreq)
+    dev = G.dev()
+    scenes = [D.Scene(dev, *p) for p in pool]
+    reqs = []
+    for k in range(nreq):
+        si = int(req_scene[k])
+        rays = builders[si].surface_rays(seed=100 + k, count=int(sizes[k]))
+        reqs.append((si, B.RTQ_TRACE_ANY if req_any[k] else B.RTQ_TRACE_CLOSEST, rays))
+    print(f"{int(sizes.sum())} rays in {nreq} requests", flush=True)
+    alone = []
+    for si, flags, rays in reqs:
+        if flags == B.RTQ_TRACE_CLOSEST:
+            g = scenes[si].trace_closest(rays)
+            alone.append((g["hits"].view(np.uint32).copy(), g["nid"]))
+        else:
+            alone.append((scenes[si].trace_any(rays), None))
+    # Poisson arrivals at 400 requests/s, spread over 4 submitting threads
+    arrivals = np.cumsum(rng.exponential(1.0 / 400.0, nreq))
+    srv = D.Server(dev, scenes, budget_bytes=2 << 30)
+    handles = [None] * nreq
+    start = time.perf_counter()
+
+    def client(t):
+        for k in range(t, nreq, 4):
+            wait = start + arrivals[k] - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            si, flags, rays = reqs[k]
+            handles[k] = srv.submit(si, flags, rays)
+
+    th = [threading.Thread(target=client, args=(t,)) for t in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    lat, batches = [], set()
+    for k, h in enumerate(handles):
+        assert h.wait() == B.RTQ_OK
+        st = h.stats()
+        lat.append(st.latency_ms)
+        batches.add(st.batch_id)
+        a_out, a_nid = alone[k]
+        if reqs[k][1] == B.RTQ_TRACE_CLOSEST:
+            assert np.array_equal(h.out.view(np.uint32), a_out), f"request {k}"
+            assert np.array_equal(h.nid, a_nid), f"request {k} nid"
+        else:
+            assert np.array_equal(h.out, a_out), f"request {k}"
+    print(f"{len(batches)} launches for {nreq} requests, p50 {np.percentile(lat, 50):.1f} ms, "
+          f"p99 {np.percentile(lat, 99):.1f} ms", flush=True)
+    assert len(batches) < nreq
+    for h in handles:
+        h.release()
+    srv.close()
+    # the oracle on strided samples of every request, one call per scene and mode
+    by_scene = {}
+    for k, (si, flags, rays) in enumerate(reqs):
+        idx = np.arange(0, rays.size, max(1, rays.size // 4096))[:4096]
+        by_scene.setdefault((si, flags), []).append((k, idx))
+    for (si, flags), parts in by_scene.items():
+        sample = np.concatenate([reqs[k][2][idx] for k, idx in parts])
+        sc = oracle.Scene(*pool[si])
+        if flags == B.RTQ_TRACE_CLOSEST:
+            o = sc.trace_closest(sample, threads=16)
+            got = np.concatenate([alone[k][0].reshape(-1, 4)[idx] for k, idx in parts])
+            nid = np.concatenate([alone[k][1][idx] for k, idx in parts])
+            assert np.array_equal(got[:, 0], o["tri_id"]) and np.array_equal(got[:, 1], o["t"].view(np.uint32))
+            assert np.array_equal(got[:, 2], o["u"].view(np.uint32)) and np.array_equal(got[:, 3], o["v"].view(np.uint32))
+            assert np.array_equal(nid, o["nid"])
+        else:
+            got = np.concatenate([alone[k][0][idx] for k, idx in parts])
+            assert np.array_equal(got, sc.trace_any(sample, threads=16)[0])

@@ -214,3 +214,94 @@ def workload(config: int, scale: float = 1.0, threads: int = 8):
         raise ValueError(config)
     b.build(threads)
     return b, f, w, h
+
+
+# ---- many-scene workloads (the batched and served forms) ------------------------------------------
+
+def loguniform(rng, lo: float, hi: float, n: int) -> np.ndarray:
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+
+
+def sphere_builders(sizes, seed0: int):
+    """One unbuilt builder per triangle count: a soup of ceil(tris / 256) seeded UV spheres of 256
+    triangles each (the spheres generator), seeds seed0 + i."""
+    out = []
+    for i, tris in enumerate(sizes):
+        b = BVHBuilder()
+        b.generate(B.RTQ_SCENE_SPHERES, seed0 + i, max(1, int(math.ceil(tris / 256))))
+        out.append(b)
+    return out
+
+
+def build_many(builders, threads: int = 16, cache_dir: str | None = None, progress=None):
+    """Builds every builder, up to `threads` at once and largest first. A scene gets one host thread,
+    or a few if it holds a large share of all triangles (so the largest scenes do not finish alone at
+    the end); results do not depend on the thread count. With cache_dir, a scene whose inputs match a
+    cached file is loaded from it instead and built scenes are stored there. Returns one
+    (blob, Ranges, positions) per builder, in order."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    tris = [builders[i].stats().num_triangles for i in range(len(builders))]
+    total = max(1, sum(tris))
+    order = sorted(range(len(builders)), key=lambda i: -tris[i])
+    out = [None] * len(builders)
+    done = [0]
+
+    def one(i):
+        b = builders[i]
+        path = None
+        if cache_dir:
+            os.makedirs(cache_dir, exist_ok=True)
+            key = b.input_hash()
+            path = os.path.join(cache_dir, f"{key:016x}.rtqbvh")
+            hit = load_cache(path, key)
+            if hit is not None:
+                out[i] = hit[:3]
+                return
+        b.build(max(1, min(threads, int(4 * threads * tris[i] / total))))
+        if path:
+            b.save_cache(path)
+        blob, ranges = b.fill_gpu_buffer()
+        out[i] = (blob, ranges, b.positions())
+        done[0] += 1
+        if progress and done[0] % 32 == 0:
+            progress(f"built {done[0]} of {len(builders)} scenes")
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(one, order))
+    return out
+
+
+def primary_rays(n: int, seed: int) -> np.ndarray:
+    """n jittered primary rays from the origin looking +x (square frame, reference field of view)."""
+    side = int(np.ceil(np.sqrt(n)))
+    f = camera_frustum((0, 0, 0), (1, 0, 0), (0, 0, 1), REFERENCE_FOV, 1.0)
+    return np.ascontiguousarray(frustum_rays(f, side, side, seed=seed)[:n])
+
+
+def batch_workload(scale: float = 1.0, slots: int = 256):
+    """The batched form's workload: `slots` scenes of LogU[1K, 2M * scale] triangles (seeds 1000 + i)
+    and per item LogU[4K, 1M] rays, primary rays for even items and diffuse surface rays for odd ones.
+    Returns (unbuilt builders, ray counts); item_rays() makes an item's rays."""
+    rng = np.random.default_rng(4)
+    sizes = loguniform(rng, 1000, 2e6 * scale, slots)
+    counts = loguniform(rng, 4096, 1 << 20, slots).astype(np.int64)
+    return sphere_builders(sizes, 1000), counts
+
+
+def batch_item_rays(builder, item: int, n: int) -> np.ndarray:
+    return primary_rays(n, 10 + item) if item % 2 == 0 else builder.surface_rays(seed=10 + item, count=n)
+
+
+def serve_workload(scale: float = 1.0, slots: int = 64, requests: int = 1000):
+    """The served form's workload: `slots` resident scenes of LogU[10K, 3M * scale] triangles (seeds
+    2000 + i) and a request stream of LogU[1K, 2M] rays each, half closest and half any hit, on uniformly
+    drawn scenes. Returns (unbuilt builders, rng, request sizes, request scenes, any-hit flags); the rng
+    continues for the arrival times."""
+    rng = np.random.default_rng(5)
+    sizes = loguniform(rng, 1e4, 3e6 * scale, slots)
+    builders = sphere_builders(sizes, 2000)
+    req_sizes = loguniform(rng, 1000, 2e6, requests).astype(np.int64)
+    req_scene = rng.integers(0, slots, requests)
+    req_any = rng.random(requests) < 0.5
+    return builders, rng, req_sizes, req_scene, req_any

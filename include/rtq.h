@@ -244,6 +244,19 @@ rtq_status rtq_cull_lights(rtq_scene* scene, const rtq_frustum* frustum, uint32_
                            uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity, uint32_t* h_total,
                            void* stream);
 
+/* rtq_cull_lights that allocates nothing. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same kernels and results as rtq_cull_lights with h_total NULL, asynchronous on
+ * `stream`; the total is d_tile_offsets[num_tiles]. Its scratch is a fixed 16 MiB device array that
+ * every GPU the library runs on holds until the process ends, so a call needs
+ * num_tiles * (2 * ceil(num_lights / 64) + 1) <= 4 Mi words (3840x2160 with 4096 lights fits) and
+ * returns RTQ_ERR_LIMIT beyond it, launching nothing. Every call on one GPU uses that GPU's one
+ * array: calls that overlap on one GPU (other streams, other threads) race and must be ordered by
+ * the caller. */
+rtq_status rtq_cull_lights_noalloc(rtq_scene* scene, const rtq_frustum* frustum, uint32_t width, uint32_t height,
+                                   uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity,
+                                   void* stream);
+
 /* ---- batched form: many scenes and ray sets in one persistent launch ----
  * No counterpart in the reference (it dispatches one pass at a time, raytracingPass.cpp:89-110).
  * Items reference resident scenes by index; each item's rays, outputs and t limits are exactly as in

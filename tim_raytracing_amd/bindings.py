@@ -34,6 +34,7 @@ RTQ_OPTION_LEAF_WAIT = 4
 RTQ_NO_HIT = 0xFFFFFFFF
 RTQ_PROBE_RAYS = 64
 RTQ_BATCH_REFILL = 1
+RTQ_ERR_LIMIT = 4
 RTQ_ERR_CACHE_MISS = 12
 RTQ_SCENE_BOX, RTQ_SCENE_SPHERES, RTQ_SCENE_SOUP = 1, 2, 3
 
@@ -163,6 +164,7 @@ _DEVICE_SIGS = {
     "rtq_trace_probe_field": ([P, C.POINTER(Lpf), P, P, U32, P, P], C.c_int),
     "rtq_shadow_masks": ([P, P, P, P, U32, P, U32, P, P], C.c_int),
     "rtq_cull_lights": ([P, C.POINTER(Frustum), U32, U32, P, P, U32, C.POINTER(U32), P], C.c_int),
+    "rtq_cull_lights_noalloc": ([P, C.POINTER(Frustum), U32, U32, P, P, U32, P], C.c_int),
     "rtq_selftest_division": ([P, U64, U64, C.POINTER(U64)], C.c_int),
     "rtq_batch_create": ([P, P, U32, P, U32, C.POINTER(P)], C.c_int),
     "rtq_batch_destroy": ([P], C.c_int),

@@ -36,7 +36,18 @@ struct rtq_device {
   size_t sortBytes = 0;
   hipEvent_t sortDone = nullptr;
   bool sortUsed = false;
+  // this GPU's copy of the no-allocation scratch array (see `scratch` below), or nullptr with the
+  // error its lookup returned
+  uint32_t* scratch = nullptr;
+  hipError_t scratchError = hipSuccess;
 };
+
+// Scratch of rtq_cull_lights_noalloc: a fixed device array, so that the call allocates nothing.
+// Every GPU the library runs on holds its own copy, at its own address, until the process ends. In
+// host code `scratch` names a host-side stand-in, not device memory: host code only ever uses the
+// address hipGetSymbolAddress returns for the current device (rtq_device::scratch).
+constexpr uint64_t kNoAllocScratchWords = uint64_t(4) << 20;  // 16 MiB
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
 
 // Work counters for persistent dispatches: a small ring so that dispatches in flight on different
 // streams do not share a counter.
@@ -104,6 +115,22 @@ bool withStack(uint32_t depth, F&& f) {
   return false;
 }
 
+// Device address of `scratch` on GPU `ordinal` (which must be current), looked up once per GPU and
+// kept for the process.
+hipError_t scratchAddress(int ordinal, uint32_t** out) {
+  static std::mutex mu;
+  static std::unordered_map<int, std::pair<uint32_t*, hipError_t>> known;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = known.find(ordinal);
+  if (it == known.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    it = known.emplace(ordinal, std::make_pair(e == hipSuccess ? static_cast<uint32_t*>(p) : nullptr, e)).first;
+  }
+  *out = it->second.first;
+  return it->second.second;
+}
+
 #define RTQ_WITH_STACK(depth, ...)                                                       \
   do {                                                                                   \
     if (!withStack((depth), __VA_ARGS__)) return fail(RTQ_ERR_LIMIT, "tree deeper than 64"); \
@@ -148,6 +175,7 @@ rtq_status rtq_device_create(int hip_ordinal, rtq_device** out) {
   rtq_device* d = new rtq_device();
   d->ordinal = hip_ordinal;
   d->numCU = prop.multiProcessorCount;
+  d->scratchError = scratchAddress(hip_ordinal, &d->scratch);
   *out = d;
   return RTQ_OK;
 }
@@ -664,13 +692,13 @@ rtq_status cullSetup(rtq_scene* s, const rtq_frustum* f, uint32_t width, uint32_
   return RTQ_OK;
 }
 
-// kCullMask -> kScan -> kCullEmit on `st` with the scratch at `scratch` (cullScratchWords words).
+// kCullMask -> kScan -> kCullEmit on `st` with their scratch at `work` (cullScratchWords words).
 hipError_t cullLaunch(rtq_scene* s, const rtq::Frustum4& fr, uint32_t ngx, uint32_t ngy, uint32_t groups,
-                      uint32_t* scratch, uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity,
+                      uint32_t* work, uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity,
                       hipStream_t st) {
   const uint32_t ntiles = ngx * ngy;
-  uint64_t* masks = reinterpret_cast<uint64_t*>(scratch);
-  uint32_t* counts = scratch + 2ull * ntiles * groups;
+  uint64_t* masks = reinterpret_cast<uint64_t*>(work);
+  uint32_t* counts = work + 2ull * ntiles * groups;
   const dim3 grid((ntiles + rtq::kCullBlock / 64 - 1) / (rtq::kCullBlock / 64)), block(rtq::kCullBlock);
   const float4* lights = reinterpret_cast<const float4*>(s->view.lights);
   hipLaunchKernelGGL(rtq::kCullMask, grid, block, 0, st, fr, lights, s->view.numLights, ngx, ngy, groups, masks, counts);
@@ -692,11 +720,11 @@ rtq_status rtq_cull_lights(rtq_scene* s, const rtq_frustum* f, uint32_t width, u
   RTQ_HIP(hipSetDevice(s->dev->ordinal));
   hipStream_t st = static_cast<hipStream_t>(stream);
   // stream-ordered scratch: the call stays asynchronous
-  uint32_t* scratch = nullptr;
-  RTQ_HIP(hipMallocAsync(reinterpret_cast<void**>(&scratch),
+  uint32_t* work = nullptr;
+  RTQ_HIP(hipMallocAsync(reinterpret_cast<void**>(&work),
                          sizeof(uint32_t) * std::max<uint64_t>(1, rtq::cullScratchWords(ntiles, groups)), st));
-  hipError_t le = cullLaunch(s, fr, ngx, ngy, groups, scratch, d_tile_offsets, d_tile_lights, capacity, st);
-  hipError_t fe = hipFreeAsync(scratch, st);
+  hipError_t le = cullLaunch(s, fr, ngx, ngy, groups, work, d_tile_offsets, d_tile_lights, capacity, st);
+  hipError_t fe = hipFreeAsync(work, st);
   if (le != hipSuccess) {
     (void)hipStreamSynchronize(st);
     return hipFail(le, "culling kernels");
@@ -708,6 +736,25 @@ rtq_status rtq_cull_lights(rtq_scene* s, const rtq_frustum* f, uint32_t width, u
   RTQ_HIP(hipStreamSynchronize(st));
   *h_total = total;
   if (total > capacity) return fail(RTQ_ERR_LIMIT, "tile light lists exceed capacity");
+  return RTQ_OK;
+}
+
+// rtq_cull_lights with the device's fixed scratch array instead of allocated scratch: no allocation,
+// no host synchronisation. Calls that overlap on one GPU share the array and race.
+rtq_status rtq_cull_lights_noalloc(rtq_scene* s, const rtq_frustum* f, uint32_t width, uint32_t height,
+                                   uint32_t* d_tile_offsets, uint32_t* d_tile_lights, uint32_t capacity,
+                                   void* stream) {
+  rtq::Frustum4 fr;
+  uint32_t ngx = 0, ngy = 0, groups = 0;
+  rtq_status cs = cullSetup(s, f, width, height, d_tile_offsets, d_tile_lights, capacity, fr, ngx, ngy, groups);
+  if (cs != RTQ_OK) return cs;
+  if (rtq::cullScratchWords(ngx * ngy, groups) > kNoAllocScratchWords)
+    return fail(RTQ_ERR_LIMIT, "tiles x (2 x light groups of 64 + 1) exceeds the 4 Mi-word no-allocation scratch");
+  if (!s->dev->scratch) return hipFail(s->dev->scratchError, "hipGetSymbolAddress(scratch)");
+  RTQ_HIP(hipSetDevice(s->dev->ordinal));
+  hipError_t le = cullLaunch(s, fr, ngx, ngy, groups, s->dev->scratch, d_tile_offsets, d_tile_lights, capacity,
+                             static_cast<hipStream_t>(stream));
+  if (le != hipSuccess) return hipFail(le, "culling kernels");
   return RTQ_OK;
 }
 

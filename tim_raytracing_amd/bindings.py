@@ -201,6 +201,11 @@ _build_lib = None
 _device_lib = None
 
 
+def bind_build_lib(path: str):
+    """Another build of the host builder library at `path` (the tests' planted-mistake builds)."""
+    return _bind(C.CDLL(path), _BUILD_SIGS)
+
+
 def build_lib():
     """The host builder library; raises if it has not been built."""
     global _build_lib

@@ -44,10 +44,12 @@ def _run(cmd):
     return r
 
 
-def build_host(out: str | None = None) -> str:
+def build_host(out: str | None = None, extra=()) -> str:
+    """extra: more compiler flags (the tests build planted-mistake variants with -DRTQ_BUILDER_PLANT=k
+    into their own directories)."""
     out = out or os.path.join(PKG, "librtq_build.so")
     tmp = out + ".tmp"
-    _run(["g++", *HOST_FLAGS, "-o", tmp, *HOST_SRC])
+    _run(["g++", *HOST_FLAGS, *extra, "-o", tmp, *HOST_SRC])
     os.replace(tmp, out)
     return out
 

@@ -17,6 +17,16 @@
 namespace rtq {
 namespace {
 
+// Planted mistakes for the builder's pins, compiled only into test builds (-DRTQ_BUILDER_PLANT=k); a
+// library built without the macro has none. 1 = axis ties won by the earlier axis, 2 = the gain reject
+// with || instead of &&, 3 = a split candidate kept on an equal score, 4 = the bisection moving away from
+// balance, 5 = a straddling item grown into the side whose volume grows more, 6 = the duplicate threshold
+// taken against the larger child volume, 7 = the light/node test inclusive (<=) instead of strict.
+#ifndef RTQ_BUILDER_PLANT
+#define RTQ_BUILDER_PLANT 0
+#endif
+constexpr int kPlant = RTQ_BUILDER_PLANT;
+
 // linalg's component min/max (timCore/linalg.h:152-153): a<b ? a : b and a<b ? b : a.
 inline float lmin(float a, float b) { return a < b ? a : b; }
 inline float lmax(float a, float b) { return a < b ? b : a; }
@@ -69,7 +79,7 @@ inline bool pointInBox(F3 p, const Box3& b) {
 inline bool sphereTouchesBox(F3 c, float r, const Box3& b) {
   F3 closest = lmin3(lmax3(c, b.mn), b.mx);
   F3 d = sub(closest, c);
-  return ldot(d, d) < r * r;
+  return kPlant == 7 ? ldot(d, d) <= r * r : ldot(d, d) < r * r;
 }
 
 // computeMaxDistanceBetweenBoxDimensions (BVHBuilder.cpp:944-950): maxelem folds from x.
@@ -237,12 +247,13 @@ bool BvhBuilder::classify(const Ctx&, Split& out, const Box3& parent, const Box3
     } else if (!inL && !inR) {
       // dropped: the reference's "floating point precision" case (:1001-1005)
     } else {
-      const float smallest = (rightVolume < leftVolume) ? rightVolume : leftVolume;  // std::min
+      const float smallest = kPlant == 6 ? (rightVolume < leftVolume ? leftVolume : rightVolume)
+                                         : (rightVolume < leftVolume) ? rightVolume : leftVolume;  // std::min
       Box3 newLeft = intersectBox(mergeBox(leftBox, aabb), parent);
       Box3 newRight = intersectBox(mergeBox(rightBox, aabb), parent);
       float diffLeft = boxVolume(newLeft) - leftVolume;
       float diffRight = boxVolume(newRight) - rightVolume;
-      if (diffLeft < diffRight) {
+      if (kPlant == 5 ? diffRight < diffLeft : diffLeft < diffRight) {
         float grow = maxDimDistance(newLeft, leftBox);
         if ((diffLeft / smallest) >= thr || grow > meanSize) {
           toBoth();
@@ -335,7 +346,7 @@ bool BvhBuilder::searchSplit(const Ctx& ctx, const TreeNode* node, const std::ve
       u32 tol = n / (16 * 1024);
       if (tol < 1) tol = 1;
       if (absDiff <= tol) break;
-      if (out.numL > out.numR)
+      if ((out.numL > out.numR) != (kPlant == 4))
         hi = newCut;
       else
         lo = newCut;
@@ -354,7 +365,8 @@ bool BvhBuilder::searchSplit(const Ctx& ctx, const TreeNode* node, const std::ve
       Box3 rightBox = {add(ext.mn, step), ext.mx};
       if (!(checkBox(leftBox) && checkBox(rightBox))) return;
       anyValid = true;
-      const long bound = best > need - 1 ? best : need - 1;
+      const long kept = (kPlant == 3 && best >= 0) ? best - 1 : best;  // plant 3: equal scores replace
+      const long bound = kept > need - 1 ? kept : need - 1;
       Split s;
       if (!classify<false>(ctx, s, ext, leftBox, rightBox, items, bound)) return;
       const long cur = long(s.uniqueL < s.uniqueR ? s.uniqueL : s.uniqueR);
@@ -424,15 +436,17 @@ void BvhBuilder::buildRec(Ctx& ctx, u32 depth, u32 numUnique, std::vector<u32>& 
     searchSplit(ctx, node, items, 0, s[0], 0);
     int cur = 0;
     for (int a = 1; a < 3; ++a)
-      if (searchSplit(ctx, node, items, a, s[a], score(cur)) && !(score(cur) > score(a))) cur = a;
+      if (searchSplit(ctx, node, items, a, s[a], score(cur)) &&
+          (kPlant == 1 ? score(a) > score(cur) : !(score(cur) > score(a))))
+        cur = a;
     best = std::move(s[cur]);
   } else {
     searchSplit(ctx, node, items, int(depth % 3), best, 0);
   }
 
   if (depth > 0) {
-    if ((n - best.numL < params_.minObjGain && n - best.numR < params_.minObjGain) || best.uniqueL == 0 ||
-        best.uniqueR == 0) {
+    const bool noGainL = n - best.numL < params_.minObjGain, noGainR = n - best.numR < params_.minObjGain;
+    if ((kPlant == 2 ? (noGainL || noGainR) : (noGainL && noGainR)) || best.uniqueL == 0 || best.uniqueR == 0) {
       fillLeaf(node, &items);
       return;
     }

@@ -20,8 +20,10 @@ TMAX = 100.0
 
 
 class BVHBuilder:
-    def __init__(self):
-        self._lib = B.build_lib()
+    def __init__(self, lib=None):
+        """lib: another build of librtq_build.so bound by bindings.bind_build_lib (tests); default the
+        package's own."""
+        self._lib = lib if lib is not None else B.build_lib()
         h = C.c_void_p()
         B.check_build("rtq_builder_create", self._lib.rtq_builder_create(C.byref(h)))
         self._h = h

@@ -75,6 +75,9 @@ def parse():
                          "the timed dispatch (-1 = auto: on for incoherent, off for primary)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent dispatch: blocks per CU (0 = library "
                                                                  "default)")
+    ap.add_argument("--batch-params", default=None,
+                    help="batch: chunk_rays,blocks_per_cu,stack_depth,leaf_wait to use instead of autotuning "
+                         "(to profile the configuration an earlier run picked)")
     return ap.parse_args()
 
 
@@ -624,8 +627,13 @@ def bench_batch(args):
         return ev0.elapsed_ms(ev1) / args.steps
 
     plain_ms = timed(lambda: batch.launch_plain(sptr))
-    print(f"plain: {plain_ms:.3f} ms per batch; autotuning", file=sys.stderr, flush=True)
-    best, tuned_ms = batch.autotune(reps=2, stream=sptr)
+    if args.batch_params:
+        ch, bl, sd, lw = (int(x) for x in args.batch_params.split(","))
+        batch.set_params(chunk_rays=ch, blocks_per_cu=bl, stack_depth=sd, leaf_wait=lw)
+        tuned_ms = 0.0
+    else:
+        print(f"plain: {plain_ms:.3f} ms per batch; autotuning", file=sys.stderr, flush=True)
+        best, tuned_ms = batch.autotune(reps=2, stream=sptr)
     batch.set_params(use_graph=1)
     batched_ms = timed(lambda: batch.launch(sptr))
     p = batch.params()
@@ -749,11 +757,14 @@ def bench_serve(args):
                 "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                 "makespan_s": round(end - start, 3), "launches": batches}
 
+    # order: warm-up (served), plain and served capacities (everything submitted at once), then plain and
+    # served at 1.5x the plain capacity; the served runs are the 1st, 3rd and 5th groups of batch launches
     run(False, None)  # warm-up: first launches of every kernel and scene
-    plain_cap = run(True, None)    # capacities: everything submitted at once
+    plain_cap = run(True, None)
     served_cap = run(False, None)
     rate = 1.5 * plain_cap["mrays_s"] * 1e6 / float(req_sizes.mean())
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    plain = run(True, arrivals)
     served = run(False, arrivals)
     checks = {}
     if not args.no_cpu_baseline:
@@ -781,7 +792,6 @@ def bench_serve(args):
                                             f"half closest, half any hit) x {passes} passes, bitstack restatement "
                                             f"of traverseBvh / traverseBvhFast, {threads} threads, {dt:.1f} s",
                                   "cpu_model": cpu_model()}
-    plain = run(True, arrivals)
     kname = f"{'kBatchRefill' if params.chunk_rays == B.RTQ_BATCH_REFILL else 'kBatchPersistent'}<{params.stack_depth}>"
     result = {
         "metric": "served aggregate Mrays/s and p50/p99 latency vs one launch per request",

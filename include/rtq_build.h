@@ -7,7 +7,10 @@
  *   rtq_builder_add_triangle_list  BVHBuilder::addTriangleList (Renderer/BVHBuilder.cpp:265-280)
  *   rtq_builder_add_sphere_light   BVHBuilder::addSphereLight (BVHBuilder.cpp:282-285)
  *   rtq_builder_set_parameters     BVHBuilder::setParameters (BVHBuilder.cpp:378-381)
- *   rtq_builder_build              BVHBuilder::build (BVHBuilder.cpp:408-461)
+ *   rtq_builder_add_blas           BVHBuilder::addBlas (BVHBuilder.cpp:288-301)
+ *   rtq_builder_set_blas_parameters the parameters of BVHBuilder::buildBlas (BVHBuilder.cpp:383-399)
+ *   rtq_builder_build              BVHBuilder::build (BVHBuilder.cpp:408-461), after buildBlas when
+ *                                  BLASes were added (BVHData::build, BVHData.cpp:28-40)
  *   rtq_builder_gpu_size           BVHBuilder::getBvhGpuSize (BVHBuilder.cpp:1185-1222)
  *   rtq_builder_fill_gpu_buffer    BVHBuilder::fillGpuBuffer (BVHBuilder.cpp:1319-1534)
  */
@@ -60,6 +63,17 @@ rtq_status rtq_builder_add_triangle_list(rtq_builder* b, uint32_t vertex_offset,
 rtq_status rtq_builder_add_sphere_light(rtq_builder* b, const float pos[3], float radius, const float color[3],
                                         float sphere_radius);
 rtq_status rtq_builder_set_parameters(rtq_builder* b, const rtq_build_params* p);
+/* Two-level scenes. A bottom-level BVH of one triangle list over this builder's vertices (indices as
+ * for rtq_builder_add_triangle_list), added as the next BLAS instance (*blas_index); a builder with
+ * BLASes builds as the reference's top-level builder: every BLAS first with the BLAS parameters
+ * (default: the application's, as rtq_builder_create sets them), then the top level over its own
+ * triangles and the BLAS instances' boxes with rtq_builder_set_parameters' values (the application
+ * uses 25, 6, 6, 1, 0.225 there, main.cpp:203-206), hoisting into a node the instances both of its
+ * children hold (BVHBuilder.cpp:905-940). The blob carries one BlasHeader per instance and the nodes
+ * of every BLAS after the top level's. */
+rtq_status rtq_builder_add_blas(rtq_builder* b, uint32_t vertex_offset, uint32_t num_triangles,
+                                const uint32_t* indices, const float color[3], uint32_t* blas_index);
+rtq_status rtq_builder_set_blas_parameters(rtq_builder* b, const rtq_build_params* p);
 /* threads <= 1: single thread. Node numbering never depends on threads. */
 rtq_status rtq_builder_build(rtq_builder* b, int threads);
 rtq_status rtq_builder_gpu_size(const rtq_builder* b, size_t* bytes);

@@ -48,6 +48,7 @@ _SIGS = {
     "orc_hits_f64": ([P, C.c_size_t, P, P, U32, P, U32, P, P, P, P], C.c_int),
     "orc_trace_closest": ([P, C.c_size_t, P, P, U32, P, U32, P, P, P, P, P, P, P, C.c_int, C.c_int], C.c_int),
     "orc_trace_any": ([P, C.c_size_t, P, P, U32, P, U32, P, P, C.c_int, C.c_int], C.c_int),
+    "orc_trace_any_tlas": ([P, C.c_size_t, P, P, U32, P, U32, P, C.c_int], C.c_int),
     "orc_brute_closest": ([P, C.c_size_t, P, P, U32, P, U32, C.c_float, P, P, P], C.c_int),
     "orc_shadow_masks": ([P, C.c_size_t, P, P, U32, P, P, P, U32, P, U32, P, C.c_int], C.c_int),
     "orc_tile_planes": ([P, U32, U32, U32, U32, P], None),
@@ -132,6 +133,18 @@ class Scene:
                                       _p(rays), n, _p(occ), _p(cnt), form, threads)
         assert rc == 0
         return occ, cnt
+
+    def trace_any_tlas(self, rays: np.ndarray, threads: int = 8, plant: int = 0) -> np.ndarray:
+        """Two-level any hit (traverseTlasFast with tlas_collide_fast): 1 = occluded. rays: rtq_ray
+        records or an (n, 8) float32 array of the same words."""
+        if rays.dtype.names is None:
+            assert rays.dtype == np.float32 and rays.ndim == 2 and rays.shape[1] == 8
+        n = rays.size if rays.dtype.names else rays.shape[0]
+        occ = np.zeros(n, np.uint32)
+        rc = lib(plant).orc_trace_any_tlas(_p(self.blob), self.blob.size, _p(self.ranges), _p(self.pos), self.nv,
+                                           _p(np.ascontiguousarray(rays)), n, _p(occ), threads)
+        assert rc == 0, "oracle rejected the two-level blob"
+        return occ
 
     def brute_closest(self, rays: np.ndarray, window: float = 1e-5):
         n = rays.size

@@ -37,7 +37,8 @@ typedef uint32_t u32;
  * 1 = tie-break to the far child (d1 <= d0), 2 = no t shrink on acceptance, 3 = u clamp dropped,
  * 4 = probe cell without the half-cell shift, 5 = probe cell not capped at resolution - 2,
  * 6 = probe position without the half-step offset, 7 = tile corners not widened by a quarter step,
- * 8 = reference storage row off by one (16 * ceil(H / 16) - y - 1). */
+ * 8 = reference storage row off by one (16 * ceil(H / 16) - y - 1), 9 = BLAS instance box accepted
+ * at 0 (>= 0 instead of > 0), 10 = a top-level leaf's own triangles tested as well. */
 static const int kPlant = ORC_PLANT;
 #else
 static const int kPlant = 0;
@@ -139,6 +140,8 @@ typedef struct {
   u32 num_lights;
   const float* pos;  /* g_positionData */
   u32 num_vertices;
+  const uint8_t* blas; /* g_blasHeader: 32 bytes each (minExtent, matId, maxExtent, rootIndex) */
+  u32 num_blas;
   /* triple -> first triangle index (open addressing) */
   u32* map_keys;     /* 3 words per bucket */
   u32* map_vals;
@@ -171,6 +174,10 @@ static int view_init(orc_view* v, const uint8_t* blob, size_t bytes, const u32* 
   v->num_lights = r[7] / 56u;
   v->pos = pos;
   v->num_vertices = nv;
+  if ((size_t)r[12] + r[13] <= bytes) {
+    v->blas = blob + r[12];
+    v->num_blas = r[13] / 32u;
+  }
   u32 cap = 16;
   while (cap < 2 * v->num_tris + 16) cap <<= 1;
   v->map_cap = cap;
@@ -379,6 +386,135 @@ static int traverse_stack(const orc_view* v, const float* o, const float* d, flo
     if (sp == 0) return 0;
     node = stack[--sp];
   }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Two-level any hit: traverseTlasFast (bvhTraversal_inline.glsl:80-151 compiled with TLAS_COLLIDE)
+ * over the top-level tree, whose leaves run tlas_collide_fast (bvhCollision.glsl:321-349): for each
+ * BLAS index listed after the leaf's triangles, CollideBox against its BlasHeader box with the fixed
+ * t limit, and if that is > 0 (strictly: a ray starting inside the box gets 0 and skips the BLAS)
+ * traverseBvhFast from the header's rootIndex, whose leaves run bvh_collide_fast. The top-level
+ * leaf's own triangles are not tested, nor are the instances hoisted into inner nodes (only leaves
+ * call tlas_collide_fast). The top-level root is node 0, with NID_LEAF_BIT when it has no children
+ * (the shader's numNodes == 1 case). A BLAS whose root has no children is walked as a leaf; the
+ * shader would descend into it from rootIndex, which carries no leaf bit, and read past the node
+ * array. */
+static int walk_any(const orc_view* v, u32 root, const float* o, const float* d, float tmax, int top);
+
+static int tlas_leaf_any(const orc_view* v, u32 nid, const float* o, const float* d, float tmax) {
+  orc_hit h;
+  memset(&h, 0, sizeof(h));
+  nid &= NID_MASK;
+  u32 off = v->nodes[16 * nid + 3];
+  if (off == 0xFFFFFFFFu || off >= v->leaf_words) return 0;
+  u32 packed = v->leaf[off], ntri = packed & 0xFFFu, nb = (packed >> 12) & 0xFFu;
+  if (kPlant == 10 && leaf_any(v, nid, o, d, tmax, &h)) return 1;
+  for (u32 i = 0; i < nb; ++i) {
+    u32 w = 1 + off + 3 * ntri + i;
+    if (w >= v->leaf_words) return 0;
+    u32 b = v->leaf[w];
+    if (b >= v->num_blas) continue;
+    const float* hb = (const float*)(v->blas + 32 * (size_t)b);
+    u32 root;
+    memcpy(&root, v->blas + 32 * (size_t)b + 28, 4);
+    float box = orc_collide_box(o, d, hb, hb + 4, tmax);
+    if (kPlant == 9 ? box >= 0.0f : box > 0.0f) {
+      if (root >= v->num_nodes) continue;
+      u32 start = ((v->nodes[16 * root + 2] & NID_MASK) == NID_MASK) ? (root | NID_LEAF_BIT) : root;
+      if (walk_any(v, start, o, d, tmax, 0)) return 1;
+    }
+  }
+  return 0;
+}
+
+static int walk_any(const orc_view* v, u32 root, const float* o, const float* d, float tmax, int top) {
+  orc_hit h;
+  memset(&h, 0, sizeof(h));
+  u32 node = root, bitstack = 0, parent = 0, sibling = 0;
+  for (;;) {
+    while (!(node & NID_LEAF_BIT)) {
+      u32 c0, c1;
+      get_children(v, node, &c0, &c1);
+      float b0mn[3], b0mx[3], b1mn[3], b1mx[3];
+      get_boxes(v, node, b0mn, b0mx, b1mn, b1mx);
+      float d0 = orc_collide_box(o, d, b0mn, b0mx, tmax);
+      float d1 = orc_collide_box(o, d, b1mn, b1mx, tmax);
+      if (d0 < 0 && d1 < 0) break;
+      bitstack <<= 1;
+      if (d0 >= 0 && d1 >= 0) {
+        node = near_is_child1(d0, d1) ? c1 : c0;
+        bitstack |= 1u;
+      } else {
+        node = (d0 >= 0) ? c0 : c1;
+      }
+    }
+    if (node & NID_LEAF_BIT) {
+      if (top ? tlas_leaf_any(v, node, o, d, tmax) : leaf_any(v, node, o, d, tmax, &h)) return 1;
+    }
+    parent = v->nodes[16 * (node & NID_MASK) + 0];
+    sibling = v->nodes[16 * (node & NID_MASK) + 1];
+    while ((bitstack & 1u) == 0) {
+      if (bitstack == 0) return 0;
+      node = parent;
+      parent = v->nodes[16 * (node & NID_MASK) + 0];
+      sibling = v->nodes[16 * (node & NID_MASK) + 1];
+      bitstack >>= 1;
+    }
+    node = sibling;
+    bitstack ^= 1u;
+  }
+}
+
+typedef struct {
+  const orc_view* v;
+  const float* rays;
+  u32* occluded;
+  u32 begin, end;
+} orc_tlas_job;
+
+static void* run_tlas(void* arg) {
+  orc_tlas_job* j = (orc_tlas_job*)arg;
+  const orc_view* v = j->v;
+  u32 root = ((v->nodes[2] & NID_MASK) == NID_MASK) ? NID_LEAF_BIT : 0u;
+  for (u32 i = j->begin; i < j->end; ++i) {
+    const float* r = j->rays + 8 * (size_t)i;
+    j->occluded[i] = walk_any(v, root, r, r + 4, r[3], 1) ? 1u : 0u;
+  }
+  return 0;
+}
+
+/* Two-level any hit for n rays (t limit = the ray's tmax). */
+int orc_trace_any_tlas(const uint8_t* blob, size_t bytes, const u32* ranges, const float* pos, u32 nv,
+                       const float* rays, u32 n, u32* occluded, int threads) {
+  orc_view view;
+  /* a blob is two-level when some leaf lists an instance (a one-level blob still carries a 32-byte
+   * placeholder BLAS section) */
+  int two_level = 0;
+  if (view_init(&view, blob, bytes, ranges, pos, nv) == 0 && view.blas && view.num_nodes) {
+    for (u32 i = 0; i < view.num_nodes && !two_level; ++i) {
+      u32 off = view.nodes[16 * i + 3];
+      if ((view.nodes[16 * i + 2] & NID_MASK) == NID_MASK && off < view.leaf_words && ((view.leaf[off] >> 12) & 0xFFu))
+        two_level = 1;
+    }
+  }
+  if (!two_level) {
+    view_free(&view);
+    return -1;
+  }
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  if ((u32)threads > n) threads = n ? (int)n : 1;
+  pthread_t th[256];
+  orc_tlas_job jobs[256];
+  for (int t = 0; t < threads; ++t) {
+    orc_tlas_job jj = {&view, rays, occluded, (u32)((uint64_t)n * t / threads), (u32)((uint64_t)n * (t + 1) / threads)};
+    jobs[t] = jj;
+  }
+  for (int t = 1; t < threads; ++t) pthread_create(&th[t], 0, run_tlas, &jobs[t]);
+  run_tlas(&jobs[0]);
+  for (int t = 1; t < threads; ++t) pthread_join(th[t], 0);
+  view_free(&view);
+  return 0;
 }
 
 /* ---------------------------------------------------------------------------------------------

@@ -253,6 +253,138 @@ void checkOrderedFlag(Built s) {
 
 }  // namespace
 
+// A two-level scene: nblas clusters of small triangles as BLAS instances plus a few top-level triangles.
+Built buildTwoLevelScene(uint32_t nblas, uint32_t trisPer, uint32_t seed) {
+  rtq_builder* b = nullptr;
+  rtq_builder_create(&b);
+  const rtq_build_params top{25, 6, 6, 1.0f, 0.225f};
+  rtq_builder_set_parameters(b, &top);
+  uint32_t state = seed * 2654435761u + 1u;
+  auto rnd = [&] {  // uniform in [-1, 1)
+    state = state * 1664525u + 1013904223u;
+    return float(state >> 8) / float(1u << 23) - 1.0f;
+  };
+  std::vector<uint32_t> idx(3 * trisPer);
+  for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
+  for (uint32_t k = 0; k < nblas; ++k) {
+    const float c[3] = {30 * rnd(), 30 * rnd(), 30 * rnd()};
+    std::vector<float> v;
+    for (uint32_t t = 0; t < trisPer; ++t) {
+      const float tc[3] = {c[0] + 3 * rnd(), c[1] + 3 * rnd(), c[2] + 3 * rnd()};
+      for (int q = 0; q < 3; ++q)
+        for (int a = 0; a < 3; ++a) v.push_back(tc[a] + 0.5f * rnd());
+    }
+    uint32_t off = 0, bi = 0;
+    rtq_builder_add_vertices(b, v.data(), uint32_t(v.size() / 3), &off);
+    rtq_builder_add_blas(b, off, trisPer, idx.data(), nullptr, &bi);
+  }
+  const std::vector<float> own = {-40, -40, -45, 40, -40, -44, 0, 40, -46, -40, 40, -45, 40, 40, -44, 0, -40, -46};
+  uint32_t off = 0;
+  rtq_builder_add_vertices(b, own.data(), 6, &off);
+  rtq_builder_add_triangle_list(b, off, 2, idx.data(), nullptr);
+  rtq_builder_build(b, 2);
+  Built out;
+  size_t bytes = 0;
+  rtq_builder_gpu_size(b, &bytes);
+  out.blob.resize(bytes);
+  rtq_builder_fill_gpu_buffer(b, out.blob.data(), bytes, &out.ranges);
+  const float* xyz = nullptr;
+  uint32_t n = 0;
+  rtq_builder_positions(b, &xyz, &n);
+  out.pos.assign(xyz, xyz + 3 * size_t(n));
+  rtq_builder_destroy(b);
+  return out;
+}
+
+// Two-level decode: every top-level inner node and every BLAS inner node is one record with the blob's
+// boxes; a top-level leaf's ref names its instance run (header boxes, the BLAS root's ref, last flag);
+// BLAS leaves name their triangle slots; each BLAS is laid out once however many leaves list it.
+void checkTwoLevel(const Built& s, uint32_t nblas) {
+  rtq::HostScene h;
+  const char* why = "";
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), uint32_t(s.pos.size() / 3), h,
+                            &why) == RTQ_OK);
+  CHECK(h.twoLevel && !h.fastBoxes);
+  const uint32_t* nodes = reinterpret_cast<const uint32_t*>(s.blob.data() + s.ranges.nodes.offset);
+  const uint32_t* leaf = reinterpret_cast<const uint32_t*>(s.blob.data() + s.ranges.leaf_data.offset);
+  const uint8_t* hdr = s.blob.data() + s.ranges.blas.offset;
+  CHECK(s.ranges.blas.size == 32 * nblas);
+  std::vector<int> blasRef(nblas, -1);
+  std::set<uint32_t> recs, slots;
+  uint32_t listed = 0;
+  // walk(ref, id, blas): blas < 0 for the top level
+  struct Item { uint32_t ref, id; int blas; };
+  std::vector<Item> st{{h.root, 0u, -1}};
+  std::set<std::pair<uint32_t, uint32_t>> blasWalked;
+  while (!st.empty()) {
+    const Item it = st.back();
+    st.pop_back();
+    const bool isLeaf = (nodes[16 * it.id + 2] & 0x3FFFFFFFu) == 0x3FFFFFFFu;
+    if (!isLeaf) {
+      CHECK(it.ref < h.nodes.size() && recs.insert(it.ref).second);
+      const rtq::DevNode& d = h.nodes[it.ref];
+      CHECK(d.d[2] == it.id);
+      const float* f = reinterpret_cast<const float*>(nodes + 16 * it.id);
+      const float want[12] = {f[4], f[5], f[12], f[6], f[7], f[13], f[8], f[9], f[14], f[10], f[11], f[15]};
+      const float got[12] = {d.a[0], d.a[1], d.a[2], d.a[3], d.b[0], d.b[1], d.b[2], d.b[3], d.c[0], d.c[1], d.c[2], d.c[3]};
+      CHECK(std::memcmp(want, got, sizeof(want)) == 0);
+      const uint32_t l = nodes[16 * it.id + 2] & 0x3FFFFFFFu;
+      st.push_back({d.d[1], l + 1, it.blas});
+      st.push_back({d.d[0], l, it.blas});
+      continue;
+    }
+    const uint32_t off = nodes[16 * it.id + 3];
+    const uint32_t ntri = off == 0xFFFFFFFFu ? 0u : leaf[off] & 0xFFFu;
+    const uint32_t nb = off == 0xFFFFFFFFu ? 0u : (leaf[off] >> 12) & 0xFFu;
+    if (it.blas < 0) {
+      if (nb == 0) {
+        CHECK(it.ref == rtq::kRefEmpty);  // the top level's own triangles are not laid out
+        continue;
+      }
+      CHECK(it.ref & rtq::kRefLeaf);
+      const uint32_t first = it.ref & ~rtq::kRefLeaf;
+      for (uint32_t k = 0; k < nb; ++k) {
+        CHECK(first + k < h.insts.size() && slots.insert(first + k).second);
+        const rtq::DevInst& in = h.insts[first + k];
+        const uint32_t b = leaf[off + 1 + 3 * ntri + k];
+        CHECK(b < nblas);
+        float hb[8];
+        uint32_t root;
+        std::memcpy(hb, hdr + 32 * b, 32);
+        std::memcpy(&root, hdr + 32 * b + 28, 4);
+        const float want[6] = {hb[0], hb[1], hb[2], hb[4], hb[5], hb[6]};
+        const float got[6] = {in.mn[0], in.mn[1], in.mn[2], in.mx[0], in.mx[1], in.mx[2]};
+        CHECK(std::memcmp(want, got, sizeof(want)) == 0);
+        CHECK(in.last == (k + 1 == nb ? 1u : 0u));
+        CHECK(blasRef[b] < 0 || uint32_t(blasRef[b]) == in.root);
+        blasRef[b] = int(in.root);
+        ++listed;
+        if (blasWalked.insert({b, root}).second) st.push_back({in.root, root, int(b)});
+      }
+      continue;
+    }
+    CHECK(nb == 0);
+    if (ntri == 0) {
+      CHECK(it.ref == rtq::kRefEmpty);
+      continue;
+    }
+    const uint32_t first = it.ref & ~rtq::kRefLeaf;
+    CHECK((it.ref & rtq::kRefLeaf) != 0);
+    for (uint32_t k = 0; k < ntri; ++k) {
+      CHECK(first + k < h.tris.size());
+      uint32_t w[4];
+      std::memcpy(w, h.tris[first + k].t2, 16);
+      CHECK(w[2] == it.id && w[3] == (k + 1 == ntri ? 1u : 0u));
+      const uint32_t* t = leaf + off + 1 + 3 * k;
+      const uint32_t v0 = t[0] + (t[1] & 0xFFFFu);
+      CHECK(std::memcmp(h.tris[first + k].t0, &s.pos[3 * size_t(v0)], 12) == 0);
+    }
+  }
+  CHECK(listed == h.insts.size() && slots.size() == h.insts.size());
+  CHECK(recs.size() == h.nodes.size());
+  CHECK(h.maxDepth <= rtq::kMaxTreeDepth);
+}
+
 int main() {
   checkScene(build(RTQ_SCENE_BOX, 1, 0));
   checkScene(build(RTQ_SCENE_SPHERES, 2, 64));
@@ -268,6 +400,9 @@ int main() {
   checkRejects(build(RTQ_SCENE_SPHERES, 4, 8));
   checkOrderedFlag(build(RTQ_SCENE_SPHERES, 5, 16));
   checkOrderedFlag(build(RTQ_SCENE_SOUP, 6, 2000));
+  checkTwoLevel(buildTwoLevelScene(20, 200, 1), 20);
+  checkTwoLevel(buildTwoLevelScene(3, 50, 2), 3);   // a top level that is one leaf
+  checkTwoLevel(buildTwoLevelScene(40, 3, 3), 40);  // BLASes that are one leaf each
   if (g_fail) return 1;
   std::puts("ok");
   return 0;

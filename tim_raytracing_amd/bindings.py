@@ -112,6 +112,8 @@ _BUILD_SIGS = {
     "rtq_builder_add_triangle_list": ([P, U32, U32, P, P], C.c_int),
     "rtq_builder_add_sphere_light": ([P, P, F, P, F], C.c_int),
     "rtq_builder_set_parameters": ([P, C.POINTER(BuildParams)], C.c_int),
+    "rtq_builder_set_blas_parameters": ([P, C.POINTER(BuildParams)], C.c_int),
+    "rtq_builder_add_blas": ([P, U32, U32, P, P, C.POINTER(U32)], C.c_int),
     "rtq_builder_build": ([P, C.c_int], C.c_int),
     "rtq_builder_gpu_size": ([P, C.POINTER(C.c_size_t)], C.c_int),
     "rtq_builder_fill_gpu_buffer": ([P, P, C.c_size_t, C.POINTER(Ranges)], C.c_int),

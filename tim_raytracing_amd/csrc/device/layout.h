@@ -43,6 +43,17 @@ struct alignas(16) DevTri {
 };
 static_assert(sizeof(DevTri) == 48, "DevTri is 48 B");
 
+// One BLAS instance of a top-level leaf, 32 B: the BlasHeader box, the device ref of the BLAS's
+// root and whether it is the leaf's last instance. A two-level scene's top-level leaf refs are
+// kRefLeaf | index of the leaf's first instance (kRefEmpty for a leaf without instances).
+struct alignas(16) DevInst {
+  float mn[3];
+  uint32_t root;
+  float mx[3];
+  uint32_t last;
+};
+static_assert(sizeof(DevInst) == 32, "DevInst is 32 B");
+
 // Sphere light for shadow rays and culling: (x, y, z, attenuation radius); kind in w of `kinds`.
 struct DevScene {
   const DevNode* nodes;
@@ -60,6 +71,8 @@ struct DevScene {
   const uint32_t* triangles; // the blob's Triangle records (3 words each), indexed by triangle id
   uint32_t topSlots;         // leading node records a kernel may read from its LDS copy (<= kTopSlots)
   const DevNode* top;        // that LDS copy, set by the kernel; nullptr outside a staging kernel
+  const DevInst* insts;      // two-level scenes: the top-level leaves' BLAS instances (else nullptr)
+  uint32_t twoLevel;         // 1: `root` is a top-level tree whose leaves hold BLAS instances
 };
 
 // Light-probe field placement as cameraPass.comp builds LightProbFieldHeader (cameraPass.comp:62-66).

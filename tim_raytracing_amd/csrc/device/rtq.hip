@@ -131,6 +131,14 @@ hipError_t scratchAddress(int ordinal, uint32_t** out) {
   return it->second.second;
 }
 
+// Two-level scenes are traced by any-hit dispatches only: the reference's closest-hit walk of a
+// top-level tree (traverseTlas / tlas_collide, bvhTraversal_inline.glsl:3-78, bvhCollision.glsl:
+// 287-319) calls HitBox, whose only definition is commented out (collision.glsl:175-195).
+rtq_status twoLevelOnlyAny() {
+  return fail(RTQ_ERR_INVALID_ARGUMENT,
+              "two-level scene: only rtq_dispatch_rays with RTQ_TRACE_ANY (the reference has no closest-hit TLAS walk)");
+}
+
 #define RTQ_WITH_STACK(depth, ...)                                                       \
   do {                                                                                   \
     if (!withStack((depth), __VA_ARGS__)) return fail(RTQ_ERR_LIMIT, "tree deeper than 64"); \
@@ -363,6 +371,7 @@ rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, 
   if ((st = uploadArray(s, h.lights, &s->view.lights)) != RTQ_OK) return bail(st);
   if ((st = uploadArray(s, h.lightKinds, &s->view.lightKind)) != RTQ_OK) return bail(st);
   if ((st = uploadArray(s, h.triangles, &s->view.triangles)) != RTQ_OK) return bail(st);
+  if (h.twoLevel && (st = uploadArray(s, h.insts, &s->view.insts)) != RTQ_OK) return bail(st);
   {
     const uint32_t* dc = nullptr;
     if ((st = uploadArray(s, std::vector<uint32_t>(kCounterRing * rtq::kXcds, 0u), &dc)) != RTQ_OK) return bail(st);
@@ -386,6 +395,8 @@ rtq_status rtq_scene_upload(rtq_device* dev, const void* blobPtr, size_t bytes, 
   s->view.orderedBoxes = h.orderedBoxes ? 1u : 0u;
   s->view.topSlots = h.topSlots;
   s->view.top = nullptr;
+  s->view.twoLevel = h.twoLevel ? 1u : 0u;
+  if (!h.twoLevel) s->view.insts = nullptr;
   s->info.num_nodes = h.numNodes;
   s->info.num_inner = h.numInner;
   s->info.num_leaf_slots = uint32_t(h.tris.size());
@@ -473,6 +484,15 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 grid(uint32_t((uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock)), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(rays);
+  if (s->view.twoLevel) {
+    if (flags != RTQ_TRACE_ANY || counters) return twoLevelOnlyAny();
+    RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
+      constexpr int K = decltype(S)::value;
+      hipLaunchKernelGGL((rtq::kTraceAnyTlas<K>), grid, block, 0, st, s->view, r, n, static_cast<uint32_t*>(out));
+    });
+    RTQ_HIP(hipGetLastError());
+    return RTQ_OK;
+  }
   const uint32_t gridWait = s->dev->leafWait ? s->dev->leafWait : rtq::kGridWait;
   const uint32_t refillWait = s->dev->leafWait ? s->dev->leafWait : rtq::kRefillWait;
   const uint32_t* perm = nullptr;
@@ -551,6 +571,7 @@ rtq_status rtq_dispatch_rays_counted(rtq_scene* scene, const rtq_ray* d_rays, ui
 rtq_status rtq_trace_camera(rtq_scene* s, const rtq_frustum* f, uint32_t width, uint32_t height, uint32_t flags,
                             rtq_hit* d_hits, uint32_t* d_leaf_nid, void* stream) {
   if (!s || !f) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene or frustum");
+  if (s->view.twoLevel) return twoLevelOnlyAny();
   if (width == 0 || height == 0) return RTQ_OK;
   if (!d_hits) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL hit buffer");
   if (flags & ~RTQ_CAMERA_REFERENCE_ROWS) return fail(RTQ_ERR_INVALID_ARGUMENT, "bad flags");
@@ -575,6 +596,7 @@ rtq_status rtq_trace_camera_record(rtq_scene* s, const rtq_frustum* f, uint32_t 
                                    uint32_t flags, const float sun_dir[3], uint32_t shadowFlags, const rtq_lpf* lpf,
                                    uint32_t* d_records, void* stream) {
   if (!s || !f) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene or frustum");
+  if (s->view.twoLevel) return twoLevelOnlyAny();
   if (width == 0 || height == 0) return RTQ_OK;
   if (!d_records) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL record buffer");
   if (flags & ~RTQ_CAMERA_REFERENCE_ROWS) return fail(RTQ_ERR_INVALID_ARGUMENT, "bad flags");
@@ -607,6 +629,7 @@ rtq_status rtq_trace_camera_record(rtq_scene* s, const rtq_frustum* f, uint32_t 
 rtq_status rtq_trace_probe_field(rtq_scene* s, const rtq_lpf* lpf, const float dirs[4 * 64], const float sun_dir[3],
                                  uint32_t shadowFlags, uint32_t* d_records, void* stream) {
   if (!s || !lpf || !dirs) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene, probe field or directions");
+  if (s->view.twoLevel) return twoLevelOnlyAny();
   if ((shadowFlags & RTQ_SHADOW_SUN) && !sun_dir) return fail(RTQ_ERR_INVALID_ARGUMENT, "sun flag without direction");
   rtq::LpfHeader h{};
   std::memcpy(&h, lpf, sizeof(h));
@@ -636,6 +659,7 @@ rtq_status rtq_trace_probe_field(rtq_scene* s, const rtq_lpf* lpf, const float d
 rtq_status rtq_shadow_masks(rtq_scene* s, const rtq_ray* d_rays, const rtq_hit* d_hits, const uint32_t* d_leaf_nid,
                             uint32_t n, const float sun_dir[3], uint32_t flags, uint32_t* d_masks, void* stream) {
   if (!s) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL scene");
+  if (s->view.twoLevel) return twoLevelOnlyAny();
   if (n == 0) return RTQ_OK;
   if (!d_rays || !d_hits || !d_leaf_nid || !d_masks) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL buffer");
   if ((flags & RTQ_SHADOW_SUN) && !sun_dir) return fail(RTQ_ERR_INVALID_ARGUMENT, "sun flag without direction");

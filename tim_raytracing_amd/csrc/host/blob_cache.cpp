@@ -49,6 +49,15 @@ u64 inputHash(const BvhBuilder& b) {
   h = hashBytes(b.triangles().data(), b.triangles().size() * sizeof(TriangleRec), h);
   h = hashBytes(b.materials().data(), b.materials().size() * sizeof(MaterialRec), h);
   h = hashBytes(b.lights().data(), b.lights().size() * sizeof(SphereLightDesc), h);
+  if (b.numBlas()) {  // two-level: the BLAS parameters and every BLAS's triangles
+    const BuildParams& q = b.blasParameters();
+    const u32 qw[4] = {q.maxDepth, q.minObjPerNode, q.minObjGain, b.numBlas()};
+    const float qf[2] = {q.expandNodeVolumeThreshold, q.expandNodeDimensionFactor};
+    h = hashBytes(qw, sizeof(qw), h);
+    h = hashBytes(qf, sizeof(qf), h);
+    for (u32 i = 0; i < b.numBlas(); ++i)
+      h = hashBytes(b.blas(i).triangles().data(), b.blas(i).triangles().size() * sizeof(TriangleRec), h);
+  }
   return h;
 }
 

@@ -4,10 +4,13 @@
 #include "bvh_builder.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <iterator>
 #include <limits>
+#include <mutex>
 #include <stdexcept>
 #include <thread>
 #include <unordered_set>
@@ -75,6 +78,27 @@ inline bool pointInBox(F3 p, const Box3& b) {
   return !(p.x < b.mn.x || p.y < b.mn.y || p.z < b.mn.z || p.x > b.mx.x || p.y > b.mx.y || p.z > b.mx.z);
 }
 
+// boxBoxCollision != Disjoint (BVHBuilder.cpp:41-47): boxes that only touch are disjoint.
+inline bool boxesOverlap(const Box3& a, const Box3& b) {
+  return !(a.mn.x >= b.mx.x || a.mn.y >= b.mx.y || a.mn.z >= b.mx.z || b.mn.x >= a.mx.x || b.mn.y >= a.mx.y ||
+           b.mn.z >= a.mx.z);
+}
+
+// The TLAS's hoisting of items common to both children (mergeCommonItems, BVHBuilder.cpp:907-935):
+// both lists are sorted, their intersection is removed from each and returned.
+std::vector<u32> mergeCommon(std::vector<u32>& left, std::vector<u32>& right) {
+  std::sort(left.begin(), left.end());
+  std::sort(right.begin(), right.end());
+  std::vector<u32> common;
+  std::set_intersection(left.begin(), left.end(), right.begin(), right.end(), std::back_inserter(common));
+  std::vector<u32> newLeft, newRight;
+  std::set_difference(left.begin(), left.end(), common.begin(), common.end(), std::back_inserter(newLeft));
+  std::set_difference(right.begin(), right.end(), common.begin(), common.end(), std::back_inserter(newRight));
+  left = std::move(newLeft);
+  right = std::move(newRight);
+  return common;
+}
+
 // sphereBoxCollision (BVHBuilder.cpp:33-39).
 inline bool sphereTouchesBox(F3 c, float r, const Box3& b) {
   F3 closest = lmin3(lmax3(c, b.mn), b.mx);
@@ -132,7 +156,7 @@ inline LeafKey leafKey(const TriangleRec& t) {
 struct BvhBuilder::TreeNode {
   Box3 extent{};
   std::unique_ptr<TreeNode> left, right;
-  std::vector<u32> tris, lights;
+  std::vector<u32> tris, lights, blas;
 };
 
 // SplitData (BVHBuilder.h:188-206), triangle lists only.
@@ -181,6 +205,17 @@ void BvhBuilder::addTriangleList(u32 vertexOffset, u32 numTriangles, const u32* 
 
 void BvhBuilder::addSphereLight(const SphereLightDesc& l) { lights_.push_back(l); }
 
+u32 BvhBuilder::addBlas(u32 vertexOffset, u32 numTriangles, const u32* idx, const MaterialRec& material) {
+  auto b = std::make_unique<BvhBuilder>();
+  b->sharedPositions_ = &positions_;
+  b->addTriangleList(vertexOffset, numTriangles, idx, material);
+  blasMaterialOffset_.push_back(u32(materials_.size()));
+  materials_.insert(materials_.end(), b->materials_.begin(), b->materials_.end());
+  instances_.push_back({u32(blas_.size()), {}});
+  blas_.push_back(std::move(b));
+  return u32(instances_.size() - 1);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Split classification: fillSplitData<FillItems> (BVHBuilder.cpp:952-1094).
 template <bool Fill>
@@ -204,9 +239,10 @@ bool BvhBuilder::classify(const Ctx&, Split& out, const Box3& parent, const Box3
       if (lo + (n - k) <= bound) return false;
     }
     ++k;
-    const bool inL = triangleBoxOverlap(v0_[id], v1_[id], v2_[id], leftBox);
-    const bool inR = triangleBoxOverlap(v0_[id], v1_[id], v2_[id], rightBox);
     const Box3& aabb = aabb_[id];
+    const bool tri = isTriangle(id);
+    const bool inL = tri ? triangleBoxOverlap(v0_[id], v1_[id], v2_[id], leftBox) : boxesOverlap(aabb, leftBox);
+    const bool inR = tri ? triangleBoxOverlap(v0_[id], v1_[id], v2_[id], rightBox) : boxesOverlap(aabb, rightBox);
 
     auto toRight = [&]() {
       out.numR++;
@@ -378,13 +414,16 @@ bool BvhBuilder::searchSplit(const Ctx& ctx, const TreeNode* node, const std::ve
     };
     processStep(keepAxis(mulS(boxDim, 0.5f), axis));
     const float delta = float(1e-5);
-    for (u32 id : items) {
-      const Box3& b = aabb_[id];
-      F3 lo3 = {b.mn.x - delta, b.mn.y - delta, b.mn.z - delta};
-      F3 hi3 = {b.mx.x + delta, b.mx.y + delta, b.mx.z + delta};
-      processStep(keepAxis(sub(lo3, ext.mn), axis));
-      processStep(keepAxis(sub(hi3, ext.mn), axis));
-    }
+    // the reference walks the BLAS instances' bounds first, then the triangles' (:1160-1181)
+    for (int pass = 0; pass < 2; ++pass)
+      for (u32 id : items) {
+        if (isTriangle(id) != (pass == 1)) continue;
+        const Box3& b = aabb_[id];
+        F3 lo3 = {b.mn.x - delta, b.mn.y - delta, b.mn.z - delta};
+        F3 hi3 = {b.mx.x + delta, b.mx.y + delta, b.mx.z + delta};
+        processStep(keepAxis(sub(lo3, ext.mn), axis));
+        processStep(keepAxis(sub(hi3, ext.mn), axis));
+      }
     // no valid candidate: `out` stays default-constructed, as in the reference
     return recorded || !anyValid;
   }
@@ -396,6 +435,7 @@ void BvhBuilder::fillLeaf(TreeNode* node, const std::vector<u32>* items) {
   if (items) {
     const size_t cap = size_t(1) << kTriangleBits;
     for (u32 id : *items) {
+      if (!isTriangle(id)) continue;
       if (node->tris.size() >= cap) break;
       node->tris.push_back(id);
     }
@@ -407,6 +447,9 @@ void BvhBuilder::fillLeaf(TreeNode* node, const std::vector<u32>* items) {
     kv.erase(last, kv.end());
     node->tris.clear();
     for (auto& p : kv) node->tris.push_back(p.second);
+    const size_t blasCap = size_t(1) << kBlasBits;
+    for (u32 id : *items)
+      if (!isTriangle(id) && node->blas.size() < blasCap) node->blas.push_back(id - u32(triangles_.size()));
   }
   const size_t lightCap = size_t(1) << kLightBits;
   for (u32 i = 0; i < lights_.size() && node->lights.size() < lightCap; ++i) {
@@ -419,6 +462,7 @@ void BvhBuilder::buildRec(Ctx& ctx, u32 depth, u32 numUnique, std::vector<u32>& 
                           bool parallel) {
   node->tris.clear();
   node->lights.clear();
+  node->blas.clear();
   const u32 n = u32(items.size());
 
   if (numUnique <= params_.minObjPerNode || depth >= params_.maxDepth) {
@@ -472,6 +516,7 @@ void BvhBuilder::buildRec(Ctx& ctx, u32 depth, u32 numUnique, std::vector<u32>& 
     buildRec(ctx, depth + 1, sd.uniqueL, sd.itemsL, L, false);
     buildRec(ctx, depth + 1, sd.uniqueR, sd.itemsR, R, false);
   }
+  if (isTlas()) node->blas = mergeCommon(L->blas, R->blas);  // BVHBuilder.cpp:905-940
 }
 
 // Node numbering of a single-thread build: the root is 0 and every split allocates its two
@@ -490,6 +535,7 @@ void BvhBuilder::flatten(std::unique_ptr<TreeNode> root) {
     fn.extent = f.t->extent;
     fn.tris = std::move(f.t->tris);
     fn.lights = std::move(f.t->lights);
+    fn.blas = std::move(f.t->blas);
     if (f.t->left) {
       int l = int(nodes_.size());
       nodes_.emplace_back();
@@ -519,19 +565,46 @@ void BvhBuilder::flatten(std::unique_ptr<TreeNode> root) {
 // BVHBuilder::build (BVHBuilder.cpp:408-461).
 void BvhBuilder::build(int threads) {
   auto t0 = std::chrono::steady_clock::now();
-  const size_t nT = triangles_.size();
+  // buildBlas (BVHBuilder.cpp:383-399): every BLAS with the BLAS parameters. The BLASes are
+  // independent, so with several threads each thread builds whole BLASes (single-threaded each; the
+  // node numbering never depends on the thread count).
+  for (auto& b : blas_) b->setParameters(blasParams_);
+  if (threads > 1 && blas_.size() > 1) {
+    std::atomic<size_t> next{0};
+    std::exception_ptr failure;
+    std::mutex failMu;
+    auto work = [&] {
+      for (size_t k; (k = next.fetch_add(1)) < blas_.size();) {
+        try {
+          blas_[k]->build(1);
+        } catch (...) {
+          std::lock_guard<std::mutex> g(failMu);
+          if (!failure) failure = std::current_exception();
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    const size_t n = std::min(blas_.size(), size_t(threads));
+    for (size_t t = 1; t < n; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (failure) std::rethrow_exception(failure);
+  } else {
+    for (auto& b : blas_) b->build(threads);
+  }
+  const size_t nT = triangles_.size(), nB = instances_.size();
   v0_.resize(nT);
   v1_.resize(nT);
   v2_.resize(nT);
-  aabb_.resize(nT);
+  aabb_.resize(nT + nB);
   const float big = std::numeric_limits<float>::max();
   Box3 tight = {{big, big, big}, {-big, -big, -big}};
   meanTriangleSize_ = 0;
   for (size_t i = 0; i < nT; ++i) {
     const TriangleRec& t = triangles_[i];
-    F3 p0 = positions_.at(t.vertexOffset + (t.index01 & 0xFFFFu));
-    F3 p1 = positions_.at(t.vertexOffset + (t.index01 >> 16));
-    F3 p2 = positions_.at(t.vertexOffset + (t.index2_matId & 0xFFFFu));
+    F3 p0 = pos().at(t.vertexOffset + (t.index01 & 0xFFFFu));
+    F3 p1 = pos().at(t.vertexOffset + (t.index01 >> 16));
+    F3 p2 = pos().at(t.vertexOffset + (t.index2_matId & 0xFFFFu));
     v0_[i] = p0;
     v1_[i] = p1;
     v2_[i] = p2;
@@ -543,40 +616,61 @@ void BvhBuilder::build(int threads) {
     tight.mx = lmax3(tight.mx, b.mx);
     meanTriangleSize_ += lsum(sub(b.mx, b.mn)) / 3;
   }
+  for (size_t i = 0; i < nB; ++i) {  // BLAS instances after the triangles (:433-438)
+    instances_[i].aabb = blas_[instances_[i].blasId]->sceneBox_;
+    aabb_[nT + i] = instances_[i].aabb;
+    tight.mn = lmin3(tight.mn, instances_[i].aabb.mn);
+    tight.mx = lmax3(tight.mx, instances_[i].aabb.mx);
+  }
   sceneBox_ = adjustBox(tight);
+  // over the builder's own triangles only: a top level without triangles divides 0 by 0 (:442), and
+  // the resulting NaN makes the growth-distance test of the split never fire
   meanTriangleSize_ /= float(nT);
   meanTriangleSize_ *= params_.expandNodeDimensionFactor;
 
-  std::vector<u32> items(nT);
-  for (size_t i = 0; i < nT; ++i) items[i] = u32(i);
+  std::vector<u32> items(nT + nB);
+  for (size_t i = 0; i < nT + nB; ++i) items[i] = u32(i);
 
   auto root = std::make_unique<TreeNode>();
   root->extent = sceneBox_;
   Ctx ctx;
   ctx.countThreads = threads > 1 ? threads : 1;
-  buildRec(ctx, 0, u32(nT), items, root.get(), threads > 1);
+  buildRec(ctx, 0, u32(nT + nB), items, root.get(), threads > 1);
   flatten(std::move(root));
   if (nodes_.size() >= kNidMask) throw std::runtime_error("node count exceeds NID_MASK");
   stats_.buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// getBvhGpuSize (BVHBuilder.cpp:1185-1222) for a scene of triangles and sphere lights.
+// getBvhGpuSize (BVHBuilder.cpp:1185-1222) for a scene of triangles, sphere lights and BLAS instances:
+// the node records of the top level and of every BLAS are each aligned on their own here, while the
+// blob stores them as one array, so the size can exceed what is written (the rest stays zero).
 size_t BvhBuilder::gpuSize() const {
+  u32 tris = u32(triangles_.size());
+  for (const auto& b : blas_) tris += b->numTriangles();
   u32 size = alignUp(u32(materials_.size() * sizeof(MaterialRec)), kBufferAlignment);
-  size += alignUp(u32(triangles_.size() * sizeof(TriangleRec)), kBufferAlignment);
+  size += alignUp(u32(tris * sizeof(TriangleRec)), kBufferAlignment);
   size += alignUp(0, kBufferAlignment);  // primitives
   size += alignUp(u32(lights_.size() * sizeof(PackedLightRec)), kBufferAlignment);
-  size += alignUp(0, kBufferAlignment);  // BLAS headers
-  size += alignUp(u32(nodes_.size() * sizeof(PackedNodeRec)), kBufferAlignment);
-  for (const auto& n : nodes_) {
-    size += u32(1 + n.lights.size()) * 4u;
-    size += u32(n.tris.size()) * u32(sizeof(TriangleRec));
-  }
+  size += alignUp(u32(instances_.size() * 32u), kBufferAlignment);  // BlasHeader
+  auto nodesSize = [](const std::vector<FlatNode>& nodes) {
+    u32 sz = alignUp(u32(nodes.size() * sizeof(PackedNodeRec)), kBufferAlignment);
+    for (const auto& n : nodes) {
+      sz += u32(1 + n.lights.size() + n.blas.size()) * 4u;
+      sz += u32(n.tris.size()) * u32(sizeof(TriangleRec));
+    }
+    return sz;
+  };
+  size += nodesSize(nodes_);
+  for (const auto& b : blas_) size += nodesSize(b->nodes_);
   return size;
 }
 
 // fillGpuBuffer (BVHBuilder.cpp:1319-1534). Bytes the reference leaves uninitialised (section
-// padding, unused light parameters) are zero here.
+// padding, unused light parameters, BlasHeader.matId) are zero here. In a top-level blob the nodes of
+// every BLAS follow the top level's, renumbered by their offset, and their leaves inline the BLAS's
+// triangle records with the BLAS's material offset, as they stand in the triangle section (the
+// reference copies m_triangles[t] of the top-level builder there, whose index range does not hold
+// them: BVHBuilder.cpp:1490-1495).
 void BvhBuilder::fillGpuBuffer(void* outPtr, BlobRanges* r) const {
   const size_t total = gpuSize();
   uint8_t* base = static_cast<uint8_t*>(outPtr);
@@ -584,12 +678,30 @@ void BvhBuilder::fillGpuBuffer(void* outPtr, BlobRanges* r) const {
   u32 off = 0;
   auto clamp32 = [](u32 s) { return s < kBufferAlignment ? kBufferAlignment : s; };
 
-  r->blas = {off, clamp32(0)};
-  off += alignUp(0, kBufferAlignment);
+  const u32 headerBytes = u32(instances_.size() * 32u);
+  r->blas = {off, clamp32(headerBytes)};
+  const u32 headerOff = off;
+  off += alignUp(headerBytes, kBufferAlignment);
 
-  u32 triBytes = u32(triangles_.size() * sizeof(TriangleRec));
+  std::vector<u32> blasTriOffset(blas_.size());
+  u32 numTris = u32(triangles_.size());
+  for (size_t i = 0; i < blas_.size(); ++i) {
+    blasTriOffset[i] = numTris;
+    numTris += blas_[i]->numTriangles();
+  }
+  auto blasTriangle = [&](size_t i, u32 t) {  // fillTriangles(out, matIdOffset) (:1536-1546)
+    TriangleRec rec = blas_[i]->triangles_[t];
+    rec.index2_matId += blasMaterialOffset_[i] << 16;
+    return rec;
+  };
+  const u32 triBytes = u32(numTris * sizeof(TriangleRec));
   r->triangles = {off, clamp32(triBytes)};
-  if (triBytes) std::memcpy(base + off, triangles_.data(), triBytes);
+  if (!triangles_.empty()) std::memcpy(base + off, triangles_.data(), triangles_.size() * sizeof(TriangleRec));
+  for (size_t i = 0; i < blas_.size(); ++i)
+    for (u32 t = 0; t < blas_[i]->numTriangles(); ++t) {
+      const TriangleRec rec = blasTriangle(i, t);
+      std::memcpy(base + off + size_t(blasTriOffset[i] + t) * sizeof(TriangleRec), &rec, sizeof(rec));
+    }
   off += alignUp(triBytes, kBufferAlignment);
 
   r->primitives = {off, clamp32(0)};
@@ -618,31 +730,44 @@ void BvhBuilder::fillGpuBuffer(void* outPtr, BlobRanges* r) const {
   }
   off += alignUp(lightBytes, kBufferAlignment);
 
-  u32 nodeBytes = u32(nodes_.size() * sizeof(PackedNodeRec));
+  // every node of the top level, then of each BLAS (merged and renumbered, :1441-1457)
+  struct Src { const FlatNode* n; u32 shift; int blas; };
+  std::vector<Src> all;
+  for (const FlatNode& n : nodes_) all.push_back({&n, 0u, -1});
+  std::vector<u32> blasRoot(blas_.size());
+  for (size_t i = 0; i < blas_.size(); ++i) {
+    blasRoot[i] = u32(all.size());
+    for (const FlatNode& n : blas_[i]->nodes_) all.push_back({&n, blasRoot[i], int(i)});
+  }
+  u32 nodeBytes = u32(all.size() * sizeof(PackedNodeRec));
   r->nodes = {off, clamp32(nodeBytes)};
   uint8_t* nodeOut = base + off;
   u32 leafBase = off + alignUp(nodeBytes, kBufferAlignment);
   u32* leafBegin = reinterpret_cast<u32*>(base + leafBase);
   u32* cur = leafBegin;
 
-  auto isLeaf = [&](int i) { return i >= 0 && nodes_[i].left < 0; };
-  for (size_t ni = 0; ni < nodes_.size(); ++ni) {
-    const FlatNode& n = nodes_[ni];
+  for (size_t ni = 0; ni < all.size(); ++ni) {
+    const FlatNode& n = *all[ni].n;
+    const std::vector<FlatNode>& tree = all[ni].blas < 0 ? nodes_ : blas_[all[ni].blas]->nodes_;
+    const u32 shift = all[ni].shift;
+    auto isLeaf = [&](int i) { return i >= 0 && tree[i].left < 0; };
     u32 leafIndex = u32(cur - leafBegin);
-    u32 packed = u32(n.tris.size()) + (0u << kTriangleBits) + (0u << (kTriangleBits + kBlasBits)) +
+    u32 packed = u32(n.tris.size()) + (u32(n.blas.size()) << kTriangleBits) + (0u << (kTriangleBits + kBlasBits)) +
                  (u32(n.lights.size()) << (kTriangleBits + kBlasBits + kPrimitiveBits));
     *cur++ = packed;
     for (u32 t : n.tris) {
-      std::memcpy(cur, &triangles_[t], sizeof(TriangleRec));
+      const TriangleRec rec = all[ni].blas < 0 ? triangles_[t] : blasTriangle(size_t(all[ni].blas), t);
+      std::memcpy(cur, &rec, sizeof(TriangleRec));
       cur += 3;
     }
+    for (u32 b : n.blas) *cur++ = b;
     for (u32 l : n.lights) *cur++ = l;
 
     // packNodeData (:1283-1317)
     PackedNodeRec rec{};
-    u32 left = n.left >= 0 ? u32(n.left) : kNidMask;
-    u32 parent = n.parent >= 0 ? u32(n.parent) : kNidMask;
-    u32 sibling = n.sibling >= 0 ? u32(n.sibling) : kNidMask;
+    u32 left = n.left >= 0 ? u32(n.left) + shift : kNidMask;
+    u32 parent = n.parent >= 0 ? u32(n.parent) + shift : kNidMask;
+    u32 sibling = n.sibling >= 0 ? u32(n.sibling) + shift : kNidMask;
     if (isLeaf(n.sibling)) sibling |= kNidLeafBit;
     u32 child = left | (isLeaf(n.left) ? kNidLeftLeafBit : 0u) | (isLeaf(n.right) ? kNidRightLeafBit : 0u);
     rec.nid[0] = parent;
@@ -650,8 +775,8 @@ void BvhBuilder::fillGpuBuffer(void* outPtr, BlobRanges* r) const {
     rec.nid[2] = child;
     rec.nid[3] = packed != 0 ? leafIndex : 0xFFFFFFFFu;
     if (n.left >= 0) {
-      const Box3& b0 = nodes_[n.left].extent;
-      const Box3& b1 = nodes_[n.right].extent;
+      const Box3& b0 = tree[n.left].extent;
+      const Box3& b1 = tree[n.right].extent;
       float n0[4] = {b0.mn.x, b0.mn.y, b0.mx.x, b0.mx.y};
       float n1[4] = {b1.mn.x, b1.mn.y, b1.mx.x, b1.mx.y};
       float nz[4] = {b0.mn.z, b0.mx.z, b1.mn.z, b1.mx.z};
@@ -662,6 +787,17 @@ void BvhBuilder::fillGpuBuffer(void* outPtr, BlobRanges* r) const {
     std::memcpy(nodeOut + ni * sizeof(PackedNodeRec), &rec, sizeof(rec));
   }
   r->leafData = {leafBase, u32(reinterpret_cast<uint8_t*>(cur) - reinterpret_cast<uint8_t*>(leafBegin))};
+
+  // BlasHeader {minExtent, matId, maxExtent, rootIndex} per instance (:1521-1533)
+  for (size_t i = 0; i < instances_.size(); ++i) {
+    const Box3& b = instances_[i].aabb;
+    const float mn[3] = {b.mn.x, b.mn.y, b.mn.z}, mx[3] = {b.mx.x, b.mx.y, b.mx.z};
+    const u32 root = blasRoot[instances_[i].blasId];
+    uint8_t* h = base + headerOff + i * 32u;
+    std::memcpy(h, mn, 12);
+    std::memcpy(h + 16, mx, 12);
+    std::memcpy(h + 28, &root, 4);
+  }
 }
 
 std::vector<u32> BvhBuilder::leafTriangleIds() const {

@@ -2,10 +2,11 @@
 // duplicating splits (Renderer/BVHBuilder.cpp) and of its flattener (fillGpuBuffer), so that the
 // device path consumes exactly the blob the reference emits.
 //
-// Scope: triangle lists and sphere lights (what the traversal hot path reads). Analytic
-// primitives, BLAS instances and triangle strips are compiled out of the reference's traced path
-// (bvhCollision.glsl:219-238, INLINE_STRIPS 0 at BVHBuilder.cpp:14) and are not built here; their
-// blob sections are emitted empty, exactly as the reference emits them for a scene without them.
+// Scope: triangle lists, sphere lights and bottom-level BVH instances (a builder given BLASes is a
+// top-level builder, as BVHBuilder(name, geometry, isTlas = true) is). Analytic primitives and
+// triangle strips are compiled out of the reference's traced path (bvhCollision.glsl:219-238,
+// INLINE_STRIPS 0 at BVHBuilder.cpp:14) and are not built here; their blob sections are emitted empty,
+// exactly as the reference emits them for a scene without them.
 #pragma once
 #include <cstddef>
 #include <memory>
@@ -54,9 +55,22 @@ class BvhBuilder {
   void setParameters(const BuildParams& p) { params_ = p; }
   const BuildParams& parameters() const { return params_; }
 
-  // BVHBuilder::build (BVHBuilder.cpp:408-461). threads > 1 builds large subtrees concurrently
-  // (the reference forks above 10 000 items, :879-903); node numbering is always the
-  // single-thread order, so the blob does not depend on threads.
+  // BVHBuilder::addBlas (BVHBuilder.cpp:288-301) for a bottom-level BVH of one triangle list over this
+  // builder's vertices (one BLAS per loaded shape, as Scene::loadBlas makes them, Scene.cpp:229-263):
+  // the BLAS's materials (its default material and the list's) are appended to this builder's and the
+  // instance is the next BLAS index. Returns that index.
+  u32 addBlas(u32 vertexOffset, u32 numTriangles, const u32* indices, const MaterialRec& material);
+  // The parameters buildBlas builds every BLAS with (BVHBuilder.cpp:383-399); setParameters sets the
+  // top level's, as BVHData::build does (BVHData.cpp:31-35).
+  void setBlasParameters(const BuildParams& p) { blasParams_ = p; }
+  bool isTlas() const { return !blas_.empty(); }
+  u32 numBlas() const { return u32(blas_.size()); }
+  const BvhBuilder& blas(u32 i) const { return *blas_[i]; }
+  const BuildParams& blasParameters() const { return blasParams_; }
+
+  // BVHBuilder::build (BVHBuilder.cpp:408-461), after buildBlas for a top-level builder. threads > 1
+  // builds large subtrees concurrently (the reference forks above 10 000 items, :879-903); node
+  // numbering is always the single-thread order, so the blob does not depend on threads.
   void build(int threads);
 
   // getBvhGpuSize (BVHBuilder.cpp:1185-1222) and fillGpuBuffer (:1319-1534).
@@ -83,7 +97,7 @@ class BvhBuilder {
   struct FlatNode {
     Box3 extent;
     int parent = -1, sibling = -1, left = -1, right = -1;
-    std::vector<u32> tris, lights;
+    std::vector<u32> tris, lights, blas;  // blas: BLAS instance indices (top-level builders)
   };
   const std::vector<FlatNode>& nodes() const { return nodes_; }
 
@@ -106,7 +120,21 @@ class BvhBuilder {
                   const std::vector<u32>& items) const;
   void flatten(std::unique_ptr<TreeNode> root);
 
+  // Items are triangles [0, numTriangles) and, in a top-level builder, BLAS instances after them.
+  bool isTriangle(u32 id) const { return id < u32(triangles_.size()); }
+  const std::vector<F3>& pos() const { return sharedPositions_ ? *sharedPositions_ : positions_; }
+
+  struct BlasInstance {
+    u32 blasId;
+    Box3 aabb;  // the BLAS's adjusted scene box (getAABB)
+  };
+
   BuildParams params_;
+  BuildParams blasParams_;
+  std::vector<std::unique_ptr<BvhBuilder>> blas_;
+  std::vector<u32> blasMaterialOffset_;  // m_blasMaterialIdOffset
+  std::vector<BlasInstance> instances_;
+  const std::vector<F3>* sharedPositions_ = nullptr;  // a BLAS reads its top-level builder's vertices
   std::vector<F3> positions_;
   std::vector<TriangleRec> triangles_;
   std::vector<MaterialRec> materials_;

@@ -38,6 +38,28 @@ rtq_status guarded(F&& f) {
 }
 }  // namespace
 
+namespace {
+rtq::BuildParams toParams(const rtq_build_params* p) {
+  rtq::BuildParams bp;
+  bp.maxDepth = p->max_depth;
+  bp.minObjPerNode = p->min_obj_per_node;
+  bp.minObjGain = p->min_obj_gain;
+  bp.expandNodeVolumeThreshold = p->expand_node_volume_threshold;
+  bp.expandNodeDimensionFactor = p->expand_node_dimension_factor;
+  return bp;
+}
+
+// Index checks shared by triangle lists and BLASes.
+rtq_status checkList(const rtq_builder* b, uint32_t vertex_offset, uint32_t num_triangles, const uint32_t* indices) {
+  const size_t nv = b->impl.positions().size();
+  for (size_t i = 0; i < size_t(num_triangles) * 3; ++i) {
+    if (indices[i] > 0xFFFFu) return fail(RTQ_ERR_INVALID_ARGUMENT, "triangle index does not fit 16 bits");
+    if (size_t(vertex_offset) + indices[i] >= nv) return fail(RTQ_ERR_INVALID_ARGUMENT, "index beyond vertex count");
+  }
+  return RTQ_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* rtq_build_last_error(void) { return g_buildError.c_str(); }
@@ -68,11 +90,8 @@ rtq_status rtq_builder_add_vertices(rtq_builder* b, const float* xyz, uint32_t c
 rtq_status rtq_builder_add_triangle_list(rtq_builder* b, uint32_t vertex_offset, uint32_t num_triangles,
                                          const uint32_t* indices, const float color[3]) {
   if (!b || (!indices && num_triangles)) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL builder or indices");
-  const size_t nv = b->impl.positions().size();
-  for (size_t i = 0; i < size_t(num_triangles) * 3; ++i) {
-    if (indices[i] > 0xFFFFu) return fail(RTQ_ERR_INVALID_ARGUMENT, "triangle index does not fit 16 bits");
-    if (size_t(vertex_offset) + indices[i] >= nv) return fail(RTQ_ERR_INVALID_ARGUMENT, "index beyond vertex count");
-  }
+  rtq_status st = checkList(b, vertex_offset, num_triangles, indices);
+  if (st != RTQ_OK) return st;
   // material ids are 16 bits (g_MaxMaterialCount, BVHBuilder.cpp:29); material 0 is the default
   if (b->impl.numMaterials() >= 0x10000u) return fail(RTQ_ERR_LIMIT, "more than 65535 triangle lists");
   return guarded([&] {
@@ -99,17 +118,38 @@ rtq_status rtq_builder_add_sphere_light(rtq_builder* b, const float pos[3], floa
   });
 }
 
+
 rtq_status rtq_builder_set_parameters(rtq_builder* b, const rtq_build_params* p) {
   if (!b || !p) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL builder or params");
-  rtq::BuildParams bp;
-  bp.maxDepth = p->max_depth;
-  bp.minObjPerNode = p->min_obj_per_node;
-  bp.minObjGain = p->min_obj_gain;
-  bp.expandNodeVolumeThreshold = p->expand_node_volume_threshold;
-  bp.expandNodeDimensionFactor = p->expand_node_dimension_factor;
-  b->impl.setParameters(bp);
+  b->impl.setParameters(toParams(p));
   b->built = false;
   return RTQ_OK;
+}
+
+rtq_status rtq_builder_set_blas_parameters(rtq_builder* b, const rtq_build_params* p) {
+  if (!b || !p) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL builder or params");
+  b->impl.setBlasParameters(toParams(p));
+  b->built = false;
+  return RTQ_OK;
+}
+
+rtq_status rtq_builder_add_blas(rtq_builder* b, uint32_t vertex_offset, uint32_t num_triangles,
+                                const uint32_t* indices, const float color[3], uint32_t* blas_index) {
+  if (!b || (!indices && num_triangles)) return fail(RTQ_ERR_INVALID_ARGUMENT, "NULL builder or indices");
+  if (num_triangles == 0) return fail(RTQ_ERR_INVALID_ARGUMENT, "a BLAS needs triangles");
+  rtq_status st = checkList(b, vertex_offset, num_triangles, indices);
+  if (st != RTQ_OK) return st;
+  // each BLAS brings two materials (its default and the list's); ids are 16 bits
+  if (b->impl.numMaterials() + 2 > 0x10000u) return fail(RTQ_ERR_LIMIT, "more than 65536 materials");
+  if (b->impl.numBlas() >= (1u << 24)) return fail(RTQ_ERR_LIMIT, "too many BLAS instances");
+  return guarded([&] {
+    float c[3] = {0.7f, 0.7f, 0.7f};
+    if (color) std::memcpy(c, color, sizeof(c));
+    const uint32_t k = b->impl.addBlas(vertex_offset, num_triangles, indices, rtq::BvhBuilder::lambertian(c[0], c[1], c[2]));
+    if (blas_index) *blas_index = k;
+    b->built = false;
+    return RTQ_OK;
+  });
 }
 
 rtq_status rtq_builder_build(rtq_builder* b, int threads) {

@@ -72,6 +72,26 @@ class BVHBuilder:
         B.check_build("rtq_builder_set_parameters", self._lib.rtq_builder_set_parameters(self._h, C.byref(bp)))
         self.built = False
 
+    # BVHBuilder::addBlas: a bottom-level BVH of one triangle list over this builder's vertices
+    def add_blas(self, vertex_offset: int, indices, color=(0.7, 0.7, 0.7)) -> int:
+        idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+        assert idx.size % 3 == 0
+        col = np.asarray(color, dtype=np.float32)
+        k = C.c_uint32()
+        B.check_build("rtq_builder_add_blas",
+                      self._lib.rtq_builder_add_blas(self._h, vertex_offset, idx.size // 3, B.ptr(idx), B.ptr(col),
+                                                     C.byref(k)))
+        self.built = False
+        return k.value
+
+    def set_blas_parameters(self, max_depth=25, min_obj_per_node=4, min_obj_gain=4,
+                            expand_node_volume_threshold=0.25, expand_node_dimension_factor=0.5):
+        bp = B.BuildParams(max_depth, min_obj_per_node, min_obj_gain,
+                           expand_node_volume_threshold, expand_node_dimension_factor)
+        B.check_build("rtq_builder_set_blas_parameters",
+                      self._lib.rtq_builder_set_blas_parameters(self._h, C.byref(bp)))
+        self.built = False
+
     def generate(self, kind: int, seed: int, param: int = 0):
         B.check_build("rtq_scene_generate", self._lib.rtq_scene_generate(self._h, kind, seed, param))
         self.built = False
@@ -307,3 +327,31 @@ def serve_workload(scale: float = 1.0, slots: int = 64, requests: int = 1000):
     req_scene = rng.integers(0, slots, requests)
     req_any = rng.random(requests) < 0.5
     return builders, rng, req_sizes, req_scene, req_any
+
+
+def two_level_scene(seed: int, n_blas: int = 24, tris_per_blas: int = 300, own_triangles: int = 40,
+                    spread: float = 30.0):
+    """A two-level scene: `n_blas` BLAS instances, each a cluster of small random triangles around its
+    own centre in [-spread, spread]^3, and `own_triangles` large triangles of the top level on the plane
+    z = -spread - 10 + 0.1 x (the reference's top-level traversal never tests those; the plane is tilted
+    because a box flat in z at |z| >= 16 stays flat after the builder's 1e-6 widening, and CollideBox
+    never accepts a flat box). The top level uses the
+    application's TLAS parameters (main.cpp:203-206: 6, 6, 1.0 and the default depth and factor), the
+    BLASes its BLAS parameters (main.cpp:197-201, the builder's defaults). Unbuilt."""
+    rng = np.random.default_rng(seed)
+    b = BVHBuilder()
+    b.set_parameters(25, 6, 6, 1.0, 0.225)
+    centres = rng.uniform(-spread, spread, (n_blas, 3)).astype(np.float32)
+    for k in range(n_blas):
+        c = centres[k] + rng.normal(size=(tris_per_blas, 1, 3)) * rng.uniform(0.5, 3.0)
+        v = (c + rng.normal(size=(tris_per_blas, 3, 3)) * 0.4).astype(np.float32).reshape(-1, 3)
+        off = b.add_vertices(v)
+        b.add_blas(off, np.arange(3 * tris_per_blas), color=rng.uniform(0.1, 0.9, 3))
+    if own_triangles:
+        z = -spread - 10.0
+        c = rng.uniform(-spread, spread, (own_triangles, 1, 2))
+        xy = c + rng.normal(size=(own_triangles, 3, 2)) * 4.0
+        v = np.concatenate([xy, z + 0.1 * xy[..., :1]], axis=2).astype(np.float32).reshape(-1, 3)
+        off = b.add_vertices(v)
+        b.add_triangle_list(off, np.arange(3 * own_triangles))
+    return b

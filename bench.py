@@ -10,11 +10,13 @@ A step is one closest-hit dispatch over all 2 073 600 primary rays with rays, BV
 resident in HBM. `value` is primary rays / step time. The shadow pass is timed separately and
 reported beside it.
 
-roofline: the traversal kernels are held by VALU instruction issue (the scene stays in the Infinity
-Cache, so HBM sees about 1 % of the bytes the rays read): achieved = the kernel's SQ_INSTS_VALU per
-launch (committed PMC summary, profiles/pmc_<workload>.json) / the launch time measured here with HIP
-events, peak = 614.4 G wave64 VALU instructions/s; traffic = HBM bytes per launch from FETCH_SIZE /
-WRITE_SIZE and hbm_frac its share of 8 TB/s. The reference-layout algorithmic bytes per ray,
+roofline: the coherent traversal kernels are held by VALU instruction issue (the L2 and Infinity Cache
+serve nearly all the bytes the rays read): achieved = the kernel's SQ_INSTS_VALU per launch (committed
+PMC summary, profiles/pmc_<workload>.json) / the launch time measured here with HIP events, peak =
+614.4 G wave64 VALU instructions/s; traffic = bytes beyond the L2 per launch (request-size counters
+TCC_EA0_RDREQ_*/WRREQ_* weighted by request size) and hbm_frac their share of 8 TB/s, an upper bound
+of the HBM load since the Infinity Cache serves part of them. Whichever fraction is higher names the
+bound. The reference-layout algorithmic bytes per ray,
 B = 32 (ray) + 16 (hit) + 64*N_inner + 4*N_leaf + 48*N_tri, with per-ray counts from the counting
 variant of the same kernel (tests check those counts equal the oracle's), are reported beside it.
 cpu_baseline: the scalar oracle (a restatement of the reference traversal) on a bounded sample of
@@ -142,12 +144,17 @@ def pmc_record(workload: str):
 
 
 def measured_traffic(workload: str, kernel: str):
-    """HBM bytes per launch from the committed PMC summary of this workload and kernel, if present:
-    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the same counters read for a copy of
-    known size."""
+    """Bytes beyond the L2 per launch (what the L2 reads from and writes to the Infinity Cache / HBM)
+    from the committed PMC summary of this workload and kernel, if present: the request-size counters
+    (TCC_EA0_RDREQ_{32B,64B,128B} and TCC_EA0_WRREQ_64B weighted by their sizes, which read a 1 GiB copy
+    as 1.000 GiB each way), else FETCH_SIZE / WRITE_SIZE scaled by what they read for that copy."""
     rec, src = pmc_record(workload)
     k = (rec or {}).get("kernels", {}).get(kernel)
-    if not k or "fetch_kib" not in k:
+    if not k:
+        return None, None
+    if "rd_bytes" in k:
+        return int(k["rd_bytes"] + k["wr_bytes"]), src
+    if "fetch_kib" not in k:
         return None, None
     b = 1024.0 * (k["fetch_kib"] * rec["fetch_scale"] + k["write_kib"] * rec["write_scale"])
     return int(b), src

@@ -34,7 +34,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liborc.so")
+# RTQ_ORACLE_LIB names another build of the oracle (the tests' sanitizer build)
+LIB = os.environ.get("RTQ_ORACLE_LIB") or os.path.join(HERE, "liborc.so")
 REF_TRIBOX = os.path.join(HERE, "_ref", "libtribox_ref.so")
 PORT_TRIBOX = os.path.join(HERE, "libtribox_port.so")
 

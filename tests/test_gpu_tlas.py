@@ -44,7 +44,7 @@ def test_two_level_any_hit_matches_oracle(seed, kw):
     want = o.trace_any_tlas(rays, threads=16)
     bad = np.flatnonzero(got != want)
     assert bad.size == 0, f"{bad.size} rays differ, first {bad[:5]}"
-    assert 0.05 < want.mean() < 0.95
+    assert want.sum() > 100 and want.mean() < 0.95  # both answers well represented
 
 
 def test_two_level_any_hit_million_triangles():

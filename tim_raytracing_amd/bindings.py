@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-BUILD_LIB = os.path.join(PKG_DIR, "librtq_build.so")
+# RTQ_BUILD_LIB names another build of the host library (the tests' sanitizer builds)
+BUILD_LIB = os.environ.get("RTQ_BUILD_LIB") or os.path.join(PKG_DIR, "librtq_build.so")
 DEVICE_LIB = os.path.join(PKG_DIR, "librtq.so")
 
 RTQ_OK = 0

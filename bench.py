@@ -47,10 +47,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads", type=int, default=0, help="host threads for build and CPU baseline")
-    ap.add_argument("--mode", choices=("primary", "incoherent", "batch", "serve", "copy"), default="primary",
+    ap.add_argument("--mode", choices=("primary", "incoherent", "batch", "serve", "tlas", "copy"), default="primary",
                     help="primary: the headline workload; incoherent: 3 Mi-tri soup, 2 diffuse bounces + tile "
                          "culling; batch: many scenes in one persistent launch vs one launch each; serve: "
-                         "Poisson request stream, continuous batching vs one launch per request")
+                         "Poisson request stream, continuous batching vs one launch per request; tlas: any hit "
+                         "over a two-level scene of 1024 BLAS instances")
     ap.add_argument("--scale", type=float, default=1.0, help="triangle-count scale for incoherent/batch/serve")
     ap.add_argument("--distinct", type=int, default=0,
                     help="batch/serve: number of distinct BVHs built; scene slots reuse them round-robin "
@@ -374,6 +375,8 @@ def main():
         return bench_incoherent(args)
     if args.mode == "copy":
         return bench_copy(args)
+    if args.mode == "tlas":
+        return bench_tlas(args)
 
     from tim_raytracing_amd import bindings as B
     from tim_raytracing_amd import device as D
@@ -586,6 +589,70 @@ def bench_copy(args):
                       "unit": "GB/s", "bytes_copied": nbytes, "ms_per_copy": round(ms, 4), "kernel": "kCopy16",
                       "steps": args.steps, "warmup": args.warmup,
                       "pinned_link_gb_s": {"h2d": h2d, "d2h": d2h, "bytes": hbytes}}), flush=True)
+
+
+def bench_tlas(args):
+    """Two-level any hit: 1024 BLAS instances of 1000 small triangles each (clusters in [-100, 100]^3) and
+    200 top-level triangles, 4 Mi rays (from afar toward the clusters, from inside BLAS boxes, onto the
+    top-level triangles, random), one rtq_dispatch_rays(RTQ_TRACE_ANY) per step."""
+    from tim_raytracing_amd import bindings as B
+    from tim_raytracing_amd import device as D
+    from tim_raytracing_amd import host as H
+
+    threads = host_threads(args.threads)
+    t0 = time.perf_counter()
+    b = H.two_level_scene(11, n_blas=1024, tris_per_blas=1000, own_triangles=200, spread=100.0)
+    b.build(threads)
+    build_s = time.perf_counter() - t0
+    dev = D.Device(0)
+    # lane refill without a sort was the fastest of the dispatch modes on this workload
+    dispatch_cfg = configure_dispatch(dev, args, "persistent", auto_sort=0)
+    blob, ranges = b.fill_gpu_buffer()
+    pos = b.positions()
+    scene = D.Scene(dev, blob, ranges, pos)
+    n = 4 << 20
+    rays = H.two_level_rays(b, 12, n).view(B.RAY_DTYPE).reshape(-1)
+    d_r, d_o = dev.upload(rays), dev.alloc(4 * n)
+    stream = dev.stream()
+    sptr = stream.ptr
+    ev0, ev1 = dev.event(), dev.event()
+
+    def step():
+        scene.dispatch(d_r.ptr, n, B.RTQ_TRACE_ANY, d_o.ptr, None, sptr)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    ev0.record(sptr)
+    for _ in range(args.steps):
+        step()
+    ev1.record(sptr)
+    ms = ev0.elapsed_ms(ev1) / args.steps
+    got = d_o.read(np.uint32, n)
+    depth = int(scene.info().max_depth)
+    kname = kernel_name(args, depth, any_hit=True)
+    result = {
+        "metric": "two-level any-hit Mrays/s", "value": round(n / (ms * 1e-3) / 1e6, 2), "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "tlas-1024-blas-any", "blas_instances": 1024, "triangles_per_blas": 1000,
+                   "top_level_triangles": 200, "rays": n, "depth": depth},
+        "occluded_frac": round(float(got.mean()), 4),
+        "roofline": roofline("tlas-1024-blas-any", kname, ms),
+        "dispatch": dispatch_cfg, "kernel": kname, "host": {"threads": threads, "build_s": round(build_s, 1)},
+    }
+    if not args.no_cpu_baseline:
+        import oracle  # the CPU baseline leg is the only place bench.py touches the oracle
+
+        sc = oracle.Scene(blob, ranges, pos)
+        want = sc.trace_any_tlas(rays, threads=threads)
+        result["parity"] = {"rays_checked": int(n), "mismatches": int((got != want).sum()),
+                            "checker": "oracle traverseTlasFast / tlas_collide_fast restatement, every ray"}
+        value, desc = timed_oracle(lambda r, th: sc.trace_any_tlas(r, threads=th), rays, threads, args.cpu_seconds,
+                                   "two-level any hit, bitstack restatement of traverseTlasFast")
+        result["cpu_baseline"] = {"value": round(value, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                                  "sample": desc, "cpu_model": cpu_model()}
+    print(json.dumps(result), flush=True)
 
 
 def bench_batch(args):

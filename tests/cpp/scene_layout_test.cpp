@@ -304,7 +304,7 @@ void checkTwoLevel(const Built& s, uint32_t nblas) {
   const char* why = "";
   CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), uint32_t(s.pos.size() / 3), h,
                             &why) == RTQ_OK);
-  CHECK(h.twoLevel && !h.fastBoxes);
+  CHECK(h.twoLevel);
   const uint32_t* nodes = reinterpret_cast<const uint32_t*>(s.blob.data() + s.ranges.nodes.offset);
   const uint32_t* leaf = reinterpret_cast<const uint32_t*>(s.blob.data() + s.ranges.leaf_data.offset);
   const uint8_t* hdr = s.blob.data() + s.ranges.blas.offset;
@@ -341,8 +341,8 @@ void checkTwoLevel(const Built& s, uint32_t nblas) {
         CHECK(it.ref == rtq::kRefEmpty);  // the top level's own triangles are not laid out
         continue;
       }
-      CHECK(it.ref & rtq::kRefLeaf);
-      const uint32_t first = it.ref & ~rtq::kRefLeaf;
+      CHECK((it.ref & rtq::kRefLeaf) && (it.ref & rtq::kRefInst));
+      const uint32_t first = it.ref & rtq::kRefIndex;
       for (uint32_t k = 0; k < nb; ++k) {
         CHECK(first + k < h.insts.size() && slots.insert(first + k).second);
         const rtq::DevInst& in = h.insts[first + k];
@@ -369,7 +369,7 @@ void checkTwoLevel(const Built& s, uint32_t nblas) {
       continue;
     }
     const uint32_t first = it.ref & ~rtq::kRefLeaf;
-    CHECK((it.ref & rtq::kRefLeaf) != 0);
+    CHECK((it.ref & rtq::kRefLeaf) != 0 && (it.ref & rtq::kRefInst) == 0);
     for (uint32_t k = 0; k < ntri; ++k) {
       CHECK(first + k < h.tris.size());
       uint32_t w[4];
@@ -383,6 +383,50 @@ void checkTwoLevel(const Built& s, uint32_t nblas) {
   CHECK(listed == h.insts.size() && slots.size() == h.insts.size());
   CHECK(recs.size() == h.nodes.size());
   CHECK(h.maxDepth <= rtq::kMaxTreeDepth);
+  // the reciprocal slab form is allowed exactly when every node and header bound qualifies
+  bool fast = true, ordered = true;
+  for (const rtq::DevNode& d : h.nodes) {
+    for (int k = 0; k < 4; ++k)
+      for (float x : {d.a[k], d.b[k], d.c[k]}) fast = fast && rtq::detail::fastBound(x);
+    ordered = ordered && d.a[0] <= d.a[3] && d.a[1] <= d.b[0] && d.a[2] <= d.b[1] && d.b[2] <= d.c[1] &&
+              d.b[3] <= d.c[2] && d.c[0] <= d.c[3];
+  }
+  for (const rtq::DevInst& in : h.insts)
+    for (float x : {in.mn[0], in.mn[1], in.mn[2], in.mx[0], in.mx[1], in.mx[2]}) fast = fast && rtq::detail::fastBound(x);
+  CHECK(h.fastBoxes == fast && h.orderedBoxes == ordered);
+}
+
+// The stack bound of the two-level walk: the deepest top-level leaf's deferred children, one resume
+// entry, then the deepest BLAS's, never exceed maxDepth.
+void checkTwoLevelStack(const Built& s) {
+  rtq::HostScene h;
+  const char* why = "";
+  CHECK(rtq::buildHostScene(s.blob.data(), s.blob.size(), &s.ranges, s.pos.data(), uint32_t(s.pos.size() / 3), h,
+                            &why) == RTQ_OK);
+  // worst case: every box passes, so the walk defers one child per inner level it descends
+  struct Item { uint32_t ref, depth; };
+  uint32_t deepest = 0;
+  std::vector<Item> st{{h.root, 0}};
+  while (!st.empty()) {
+    const Item it = st.back();
+    st.pop_back();
+    if (it.ref == rtq::kRefEmpty) continue;
+    if (it.ref & rtq::kRefLeaf) {
+      if (it.ref & rtq::kRefInst) {
+        for (uint32_t k = it.ref & rtq::kRefIndex;; ++k) {
+          const uint32_t extra = h.insts[k].last ? 0u : 1u;
+          st.push_back({h.insts[k].root, it.depth + extra});
+          if (h.insts[k].last) break;
+        }
+      }
+      deepest = std::max(deepest, it.depth);
+      continue;
+    }
+    const rtq::DevNode& d = h.nodes[it.ref];
+    st.push_back({d.d[0], it.depth + 1});
+    st.push_back({d.d[1], it.depth + 1});
+  }
+  CHECK(deepest <= h.maxDepth);
 }
 
 int main() {
@@ -403,6 +447,8 @@ int main() {
   checkTwoLevel(buildTwoLevelScene(20, 200, 1), 20);
   checkTwoLevel(buildTwoLevelScene(3, 50, 2), 3);   // a top level that is one leaf
   checkTwoLevel(buildTwoLevelScene(40, 3, 3), 40);  // BLASes that are one leaf each
+  checkTwoLevelStack(buildTwoLevelScene(20, 200, 1));
+  checkTwoLevelStack(buildTwoLevelScene(3, 50, 2));
   if (g_fail) return 1;
   std::puts("ok");
   return 0;

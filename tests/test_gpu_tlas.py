@@ -1,5 +1,6 @@
-"""GPU parity of two-level any hit: rtq_dispatch_rays(RTQ_TRACE_ANY) on a two-level scene (kTraceAnyTlas)
-against the oracle's traverseTlasFast / tlas_collide_fast restatement on the same blob and rays.
+"""GPU parity of two-level any hit: rtq_dispatch_rays(RTQ_TRACE_ANY) on a two-level scene (the any-hit
+kernels entering BLAS instances at top-level leaves) against the oracle's traverseTlasFast /
+tlas_collide_fast restatement on the same blob and rays, in every dispatch mode.
 
 Bar: identical occlusion flags on every ray. Scenes: 24 BLAS clusters with top-level triangles, a top
 level that is one leaf, BLASes that are one leaf each, and 1024 BLAS instances of 1000 triangles (about
@@ -45,6 +46,26 @@ def test_two_level_any_hit_matches_oracle(seed, kw):
     bad = np.flatnonzero(got != want)
     assert bad.size == 0, f"{bad.size} rays differ, first {bad[:5]}"
     assert want.sum() > 100 and want.mean() < 0.95  # both answers well represented
+
+
+@pytest.mark.parametrize("persistent,sort", [(0, 1), (1, 0), (1, 1), (2, 0), (2, 1)])
+def test_two_level_dispatch_modes_identical(persistent, sort):
+    _print(f"case: two-level any hit, dispatch persistent={persistent} sort={sort}, 24 BLAS, 40000 rays")
+    d = G.dev()
+    keep = (d.get_option(B.RTQ_OPTION_PERSISTENT), d.get_option(B.RTQ_OPTION_SORT_RAYS))
+    d.set_option(B.RTQ_OPTION_PERSISTENT, persistent)
+    d.set_option(B.RTQ_OPTION_SORT_RAYS, sort)
+    try:
+        b, g, o = _scene(3)
+        rays = _records(T.scene_rays(b, 9, 40000))
+        want = o.trace_any_tlas(rays, threads=16)
+        for n in (rays.size, 37, 1):
+            got = g.trace_any(np.ascontiguousarray(rays[:n]))
+            bad = np.flatnonzero(got != want[:n])
+            assert bad.size == 0, f"{n} rays: {bad.size} differ, first {bad[:5]}"
+    finally:
+        d.set_option(B.RTQ_OPTION_PERSISTENT, keep[0])
+        d.set_option(B.RTQ_OPTION_SORT_RAYS, keep[1])
 
 
 def test_two_level_any_hit_million_triangles():

@@ -109,30 +109,7 @@ def enumerate_occlusion(b, rays):
 
 
 def scene_rays(b, seed, n):
-    """Rays of four kinds: from outside toward the BLAS clusters, from inside BLAS header boxes, down onto
-    the top level's own triangles, and random ones."""
-    _, _, nodes, leaf, hdr, _ = sections(b)
-    rng = np.random.default_rng(seed)
-    pos = b.positions()
-    hf = hdr.view(F32)
-    q = n // 4
-    rays = np.zeros((n, 8), F32)
-    rays[:, 3] = 1e4
-    centres = (hf[:, 0:3] + hf[:, 4:7]) / 2
-    org = rng.normal(size=(q, 3)) * 60
-    tgt = centres[rng.integers(0, len(hdr), q)] + rng.normal(size=(q, 3))
-    rays[:q, 0:3], rays[:q, 4:7] = org, tgt - org
-    k = rng.integers(0, len(hdr), q)
-    org = hf[k, 0:3] + rng.uniform(0.05, 0.95, (q, 3)) * (hf[k, 4:7] - hf[k, 0:3])
-    tgt = centres[rng.integers(0, len(hdr), q)] + rng.normal(size=(q, 3))
-    rays[q:2 * q, 0:3], rays[q:2 * q, 4:7] = org, tgt - org + rng.normal(size=(q, 3)) * 0.1
-    z = pos[:, 2].min() + 8.0
-    org = np.column_stack([rng.uniform(-35, 35, (q, 2)), np.full(q, z)])
-    rays[2 * q:3 * q, 0:3], rays[2 * q:3 * q, 4:7] = org, np.column_stack([rng.normal(size=(q, 2)) * 0.2, -np.ones(q)])
-    r = n - 3 * q
-    rays[3 * q:, 0:3], rays[3 * q:, 4:7] = rng.normal(size=(r, 3)) * 40, rng.normal(size=(r, 3))
-    rays[:, 4:7] /= np.linalg.norm(rays[:, 4:7], axis=1, keepdims=True)
-    return np.ascontiguousarray(rays)
+    return H.two_level_rays(b, seed, n)
 
 
 @pytest.fixture(scope="module")

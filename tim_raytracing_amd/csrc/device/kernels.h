@@ -238,6 +238,39 @@ __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, 
   }
 }
 
+// ---- two-level any hit (traverseTlasFast, bvhTraversal_inline.glsl:80-151 with TLAS_COLLIDE) -------
+// The top-level tree and every BLAS are inner-node records of one array, walked by the same descent.
+// At a top-level leaf, tlas_collide_fast (bvhCollision.glsl:321-349) takes each listed instance whose
+// BlasHeader box CollideBox reports strictly positive (> 0: a ray starting inside the box gets 0 and
+// skips that BLAS) and walks the BLAS from its root; the leaf's own triangles are never tested. Here a
+// lane reaching an instance ref (kRefLeaf | kRefInst | k) scans the leaf's instances from k for the
+// next accepted one, leaves a resume ref for the instances after it on its stack, and continues with
+// that BLAS's root; the walk of the BLAS pops back to the resume ref when it is done. An any-hit
+// answer (is some triangle reachable this way accepted?) does not depend on the visit order. The
+// slab division takes the reciprocal form when the lane's ray and the scene qualify (the header boxes
+// are part of that check), with the same results as the IEEE division.
+template <int STACK>
+__device__ __forceinline__ uint32_t enterInstance(const DevScene& s, uint32_t* stk, int& sp, uint32_t node,
+                                                  const Ray3& r, const RayRcp& rq, bool fast, float tmax) {
+  for (uint32_t k = node & kRefIndex;; ++k) {
+    const float4* ip = reinterpret_cast<const float4*>(s.insts + k);
+    const float4 lo = ip[0], hi = ip[1];
+    const float t = fast ? boxHitFast(r, rq, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, tmax)
+                         : boxHit(r, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, tmax);
+    const bool last = __float_as_uint(hi.w) != 0u;
+    if (t > 0.0f) {
+      if (!last) {
+        if (sp >= STACK) return kRefDone;  // unreachable: STACK covers both levels and the resume entry
+        stk[sp * kTraceBlock] = kRefLeaf | kRefInst | (k + 1u);
+        ++sp;
+      }
+      return __float_as_uint(lo.w);
+    }
+    if (last) break;
+  }
+  return sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
+}
+
 // Walk the tree for one ray. ANY: stop at the first accepted triangle (returns true).
 // `stk` points at this lane's column of the block's LDS stack (stride kTraceBlock).
 template <bool ANY, bool COUNT, int STACK, bool FAST, int OCT = -1>
@@ -253,6 +286,11 @@ __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const
                                           wait, 0u, cnt.inner);
     if (node == kRefDone) break;
     if (node < kRefDone) continue;  // still descending: next round
+    if (ANY && s.twoLevel && node != kRefEmpty && (node & kRefInst)) {
+      node = enterInstance<STACK>(s, stk, sp, node, r, rq, FAST, tmax);
+      if (node == kRefDone) break;
+      continue;
+    }
     if (COUNT) cnt.leaves++;
     if (node != kRefEmpty) {
       uint32_t slot = node & ~kRefLeaf;
@@ -364,108 +402,6 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s0, const floa
   }
 }
 
-// ---- two-level any hit (traverseTlasFast, bvhTraversal_inline.glsl:80-151 with TLAS_COLLIDE) -------
-// The top-level walk tests child boxes against the ray's fixed t limit like traverseBvhFast; at a
-// top-level leaf, tlas_collide_fast (bvhCollision.glsl:321-349) takes each listed instance whose
-// BlasHeader box CollideBox reports strictly positive (> 0: a ray starting inside the box gets 0 and
-// skips that BLAS) and runs traverseBvhFast in the BLAS from its root; the first accepted triangle ends
-// the ray. The answer (is any triangle reachable this way accepted?) does not depend on the visit
-// order. The BLAS walk keeps its deferred children on the same per-lane stack above the top level's
-// (STACK >= top-level depth + deepest BLAS, stack_class.h).
-template <int STACK>
-__device__ __forceinline__ bool blasAny(const DevScene& s, uint32_t* stk, int base, const Ray3& r, float tmax,
-                                        uint32_t root) {
-  uint32_t node = root;
-  int sp = base;
-  for (;;) {
-    while (node < kRefDone) {
-      const float4* np = reinterpret_cast<const float4*>(s.nodes + node);
-      const float4 a = np[0], b = np[1], c = np[2];
-      const uint4 d = reinterpret_cast<const uint4*>(np)[3];
-      const float d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, tmax);
-      const float d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, tmax);
-      const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
-      if (h0 && h1) {
-        if (sp >= STACK) return false;  // unreachable: STACK covers both levels
-        const bool n1 = d1 < d0;
-        stk[sp * kTraceBlock] = n1 ? d.x : d.y;
-        ++sp;
-        node = n1 ? d.y : d.x;
-      } else if (h0 || h1) {
-        node = h0 ? d.x : d.y;
-      } else {
-        node = sp > base ? stk[(--sp) * kTraceBlock] : kRefDone;
-      }
-    }
-    if (node == kRefDone) return false;
-    if (node != kRefEmpty) {
-      for (uint32_t slot = node & ~kRefLeaf;; ++slot) {
-        const float4* tp = reinterpret_cast<const float4*>(s.tris + slot);
-        const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-        float u, v;
-        if (triHit(r, t0, t1, t2, tmax, u, v) > 0.0f) return true;
-        if (__float_as_uint(t2.w) != 0u) break;
-      }
-    }
-    node = sp > base ? stk[(--sp) * kTraceBlock] : kRefDone;
-    if (node == kRefDone) return false;
-  }
-}
-
-template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void kTraceAnyTlas(DevScene s, const float4* __restrict__ rays, uint32_t n,
-                                                             uint32_t* __restrict__ occluded) {
-  __shared__ uint32_t lds[STACK * kTraceBlock];
-  uint32_t* stk = lds + threadIdx.x;
-  const uint32_t i = xcdBlock() * kTraceBlock + threadIdx.x;
-  if (i >= n) return;
-  float tmax;
-  const Ray3 r = loadRay(rays, i, tmax);
-  bool hit = false;
-  uint32_t node = s.root;
-  int sp = 0;
-  for (;;) {
-    while (node < kRefDone) {
-      const float4* np = reinterpret_cast<const float4*>(s.nodes + node);
-      const float4 a = np[0], b = np[1], c = np[2];
-      const uint4 d = reinterpret_cast<const uint4*>(np)[3];
-      const float d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, tmax);
-      const float d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, tmax);
-      const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
-      if (h0 && h1) {
-        if (sp >= STACK) {  // unreachable: STACK covers both levels
-          node = kRefDone;
-          break;
-        }
-        const bool n1 = d1 < d0;
-        stk[sp * kTraceBlock] = n1 ? d.x : d.y;
-        ++sp;
-        node = n1 ? d.y : d.x;
-      } else if (h0 || h1) {
-        node = h0 ? d.x : d.y;
-      } else {
-        node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
-      }
-    }
-    if (node == kRefDone) break;
-    if (node != kRefEmpty) {
-      for (uint32_t k = node & ~kRefLeaf;; ++k) {
-        const DevInst in = s.insts[k];
-        if (boxHit(r, in.mn[0], in.mn[1], in.mn[2], in.mx[0], in.mx[1], in.mx[2], tmax) > 0.0f &&
-            blasAny<STACK>(s, stk, sp, r, tmax, in.root)) {
-          hit = true;
-          break;
-        }
-        if (in.last) break;
-      }
-      if (hit) break;
-    }
-    node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
-    if (node == kRefDone) break;
-  }
-  occluded[i] = hit ? 1u : 0u;
-}
-
 // ---- persistent traversal with lane refill ------------------------------------------------
 // A fixed grid of waves; each lane traces one ray at a time. After every round of descent and leaf
 // tests the lanes whose ray has finished take new rays, so lanes no longer idle while the slowest ray
@@ -563,7 +499,9 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
       descendRound<STACK, true, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
     else
       descendRound<STACK, false, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
-    if (node > kRefDone) {
+    if (ANY && s.twoLevel && node > kRefDone && node != kRefEmpty && (node & kRefInst)) {
+      node = enterInstance<STACK>(s, stk, sp, node, r, rq, fast, tmax);
+    } else if (node > kRefDone) {
       if (node != kRefEmpty) {
         uint32_t slot = node & ~kRefLeaf;
         for (;;) {

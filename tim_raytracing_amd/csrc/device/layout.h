@@ -31,6 +31,11 @@ constexpr uint32_t kTopSlots = RTQ_TOP_SLOTS;
 constexpr uint32_t kRefLeaf = 0x80000000u;
 constexpr uint32_t kRefEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kRefDone = 0x7FFFFFFFu;  // traversal finished (never a valid inner index)
+// Two-level scenes: a top-level leaf's ref is kRefLeaf | kRefInst | index of the next instance to
+// consider (its first one when the walk reaches the leaf); every other leaf ref then has a slot below
+// 2^30, so bit 30 tells the two kinds apart.
+constexpr uint32_t kRefInst = 0x40000000u;
+constexpr uint32_t kRefIndex = 0x3FFFFFFFu;
 
 struct alignas(16) DevNode {
   float a[4], b[4], c[4];
@@ -45,7 +50,8 @@ static_assert(sizeof(DevTri) == 48, "DevTri is 48 B");
 
 // One BLAS instance of a top-level leaf, 32 B: the BlasHeader box, the device ref of the BLAS's
 // root and whether it is the leaf's last instance. A two-level scene's top-level leaf refs are
-// kRefLeaf | index of the leaf's first instance (kRefEmpty for a leaf without instances).
+// kRefLeaf | kRefInst | index of the leaf's first instance (kRefEmpty for a leaf without instances);
+// the instances of one leaf are consecutive.
 struct alignas(16) DevInst {
   float mn[3];
   uint32_t root;

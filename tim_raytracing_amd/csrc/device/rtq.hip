@@ -484,15 +484,8 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 grid(uint32_t((uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock)), block(rtq::kTraceBlock);
   const float4* r = reinterpret_cast<const float4*>(rays);
-  if (s->view.twoLevel) {
-    if (flags != RTQ_TRACE_ANY || counters) return twoLevelOnlyAny();
-    RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
-      constexpr int K = decltype(S)::value;
-      hipLaunchKernelGGL((rtq::kTraceAnyTlas<K>), grid, block, 0, st, s->view, r, n, static_cast<uint32_t*>(out));
-    });
-    RTQ_HIP(hipGetLastError());
-    return RTQ_OK;
-  }
+  // two-level scenes take the any-hit kernels below, whose leaf phase enters BLAS instances
+  if (s->view.twoLevel && (flags != RTQ_TRACE_ANY || counters)) return twoLevelOnlyAny();
   const uint32_t gridWait = s->dev->leafWait ? s->dev->leafWait : rtq::kGridWait;
   const uint32_t refillWait = s->dev->leafWait ? s->dev->leafWait : rtq::kRefillWait;
   const uint32_t* perm = nullptr;

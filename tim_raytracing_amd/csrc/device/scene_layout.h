@@ -99,6 +99,26 @@ inline rtq_status appendLeafSlots(const uint32_t* nodes, const uint32_t* leaf, u
 // traced, and are not laid out. A BLAS whose root has no children is laid out as a leaf (the shader
 // would descend into it as an inner node from rootIndex, which has no leaf bit). maxDepth is the
 // top level's depth plus the deepest BLAS's: a walk's stack holds both at a top-level leaf.
+// The reciprocal slab form needs every box bound to be 0 or of magnitude in [2^-37, 2^59].
+inline bool fastBound(float x) {
+  const float ax = std::fabs(x);
+  return ax == 0.0f || (ax >= 0x1p-37f && ax <= 0x1p59f);
+}
+// fastBoxes and orderedBoxes over the inner-node records: box 0 = min (a0, a1, a2), max (a3, b0, b1);
+// box 1 = min (b2, b3, c0), max (c1, c2, c3).
+inline void boxFlags(HostScene& h) {
+  h.fastBoxes = true;
+  h.orderedBoxes = true;
+  for (const DevNode& d : h.nodes) {
+    for (int k = 0; k < 4; ++k)
+      for (float x : {d.a[k], d.b[k], d.c[k]})
+        if (!fastBound(x)) h.fastBoxes = false;
+    if (!(d.a[0] <= d.a[3] && d.a[1] <= d.b[0] && d.a[2] <= d.b[1] && d.b[2] <= d.c[1] && d.b[3] <= d.c[2] &&
+          d.c[0] <= d.c[3]))
+      h.orderedBoxes = false;
+  }
+}
+
 inline rtq_status buildTwoLevel(const uint8_t* blob, size_t bytes, const rtq_ranges* ranges, const float* positions,
                                 uint32_t numVertices, HostScene& out, const char** err) {
   auto fail = [&](rtq_status s, const char* m) {
@@ -109,8 +129,6 @@ inline rtq_status buildTwoLevel(const uint8_t* blob, size_t bytes, const rtq_ran
     return fail(RTQ_ERR_BAD_BLOB, "the BLAS header range points outside the blob");
   HostScene h;
   h.twoLevel = true;
-  h.fastBoxes = false;
-  h.orderedBoxes = false;
   const uint32_t numNodes = ranges->nodes.size / 64u;
   const uint32_t leafWords = ranges->leaf_data.size / 4u;
   const uint32_t numTris = ranges->triangles.size / 12u;
@@ -224,16 +242,24 @@ inline rtq_status buildTwoLevel(const uint8_t* blob, size_t bytes, const rtq_ran
       in.root = it->second;
       in.last = k + 1 == nb ? 1u : 0u;
     }
-    ref = kRefLeaf | uint32_t(h.insts.size());
+    if (h.insts.size() + list.size() > kRefIndex) return fail(RTQ_ERR_LIMIT, "more than 2^30 instance entries");
+    ref = kRefLeaf | kRefInst | uint32_t(h.insts.size());
     h.insts.insert(h.insts.end(), list.begin(), list.end());
     return RTQ_OK;
   };
   uint32_t topDepth = 0;
   rtq_status st = layoutTree(0u, topLeaf, h.root, topDepth);
   if (st != RTQ_OK) return st;
-  h.maxDepth = topDepth + blasDepth;
-  if (h.maxDepth > 64) return fail(RTQ_ERR_LIMIT, "top-level plus BLAS depth above 64");
+  // a lane's stack holds the top-level walk's deferred children, one resume entry for the rest of the
+  // current leaf's instances, and the BLAS walk's deferred children
+  h.maxDepth = topDepth + 1 + blasDepth;
+  if (h.maxDepth > 64) return fail(RTQ_ERR_LIMIT, "top-level plus BLAS depth above 63");
+  if (h.tris.size() > kRefIndex) return fail(RTQ_ERR_LIMIT, "more than 2^30 triangle slots in a two-level scene");
   h.numInner = uint32_t(h.nodes.size());
+  boxFlags(h);
+  for (const DevInst& in : h.insts)
+    for (float x : {in.mn[0], in.mn[1], in.mn[2], in.mx[0], in.mx[1], in.mx[2]})
+      if (!fastBound(x)) h.fastBoxes = false;
   h.leafOff.resize(numNodes);
   for (uint32_t id = 0; id < numNodes; ++id) h.leafOff[id] = nodes[16 * id + 3];
   h.leafData.assign(leaf, leaf + leafWords);
@@ -384,18 +410,7 @@ inline rtq_status buildHostScene(const uint8_t* blob, size_t bytes, const rtq_ra
     d.d[2] = id;
     d.d[3] = 0;
   }
-  // the reciprocal slab form needs every box bound to be 0 or of magnitude in [2^-37, 2^59]
-  for (const DevNode& d : h.nodes)
-    for (int k = 0; k < 4; ++k)
-      for (float x : {d.a[k], d.b[k], d.c[k]}) {
-        const float ax = std::fabs(x);
-        if (!(ax == 0.0f || (ax >= 0x1p-37f && ax <= 0x1p59f))) h.fastBoxes = false;
-      }
-  // box 0 = min (a0, a1, a2), max (a3, b0, b1); box 1 = min (b2, b3, c0), max (c1, c2, c3)
-  for (const DevNode& d : h.nodes)
-    if (!(d.a[0] <= d.a[3] && d.a[1] <= d.b[0] && d.a[2] <= d.b[1] && d.b[2] <= d.c[1] && d.b[3] <= d.c[2] &&
-          d.c[0] <= d.c[3]))
-      h.orderedBoxes = false;
+  detail::boxFlags(h);
   h.leafOff.resize(numNodes);
   for (uint32_t id = 0; id < numNodes; ++id) h.leafOff[id] = nodes[16 * id + 3];
   h.leafData.assign(leaf, leaf + leafWords);

@@ -355,3 +355,32 @@ def two_level_scene(seed: int, n_blas: int = 24, tris_per_blas: int = 300, own_t
         off = b.add_vertices(v)
         b.add_triangle_list(off, np.arange(3 * own_triangles))
     return b
+
+
+def two_level_rays(builder, seed: int, n: int) -> np.ndarray:
+    """(n, 8) float32 rays (rtq_ray words, t limit 1e4) for a built two_level_scene, four kinds in equal
+    parts: from afar toward the BLAS clusters, from inside BLAS header boxes toward other clusters, down
+    onto the top level's own triangles, and random ones."""
+    blob, r = builder.fill_gpu_buffer()
+    hf = blob[r.blas.offset:r.blas.offset + r.blas.size].view(np.float32).reshape(-1, 8)
+    pos = builder.positions()
+    rng = np.random.default_rng(seed)
+    q = n // 4
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 3] = 1e4
+    lo, hi = pos.min(0), pos.max(0)
+    span = float((hi - lo).max())
+    centres = (hf[:, 0:3] + hf[:, 4:7]) / 2
+    org = rng.normal(size=(q, 3)) * span
+    tgt = centres[rng.integers(0, len(hf), q)] + rng.normal(size=(q, 3))
+    rays[:q, 0:3], rays[:q, 4:7] = org, tgt - org
+    k = rng.integers(0, len(hf), q)
+    org = hf[k, 0:3] + rng.uniform(0.05, 0.95, (q, 3)) * (hf[k, 4:7] - hf[k, 0:3])
+    tgt = centres[rng.integers(0, len(hf), q)] + rng.normal(size=(q, 3))
+    rays[q:2 * q, 0:3], rays[q:2 * q, 4:7] = org, tgt - org + rng.normal(size=(q, 3)) * 0.1
+    org = np.column_stack([rng.uniform(lo[0], hi[0], q), rng.uniform(lo[1], hi[1], q), np.full(q, lo[2] + 8.0)])
+    rays[2 * q:3 * q, 0:3], rays[2 * q:3 * q, 4:7] = org, np.column_stack([rng.normal(size=(q, 2)) * 0.2, -np.ones(q)])
+    m = n - 3 * q
+    rays[3 * q:, 0:3], rays[3 * q:, 4:7] = rng.normal(size=(m, 3)) * (span / 1.5), rng.normal(size=(m, 3))
+    rays[:, 4:7] /= np.linalg.norm(rays[:, 4:7], axis=1, keepdims=True)
+    return np.ascontiguousarray(rays)

@@ -160,7 +160,10 @@ rtq_status rtq_device_copy(rtq_device* dev, void* d_dst, const void* d_src, size
  * (BVHBuilder.cpp:1319-1534); `positions_xyz` holds the BVHGeometry vertex positions (12 B each).
  * The blob is converted once into the device layout and stays resident in HBM until
  * rtq_scene_destroy. Fails with RTQ_ERR_BAD_BLOB if ranges/offsets point outside the blob or a
- * triangle references a vertex >= num_vertices, and RTQ_ERR_LIMIT if the tree is deeper than 64. */
+ * triangle references a vertex >= num_vertices, and RTQ_ERR_LIMIT if the tree is deeper than 64.
+ * A blob whose leaves list BLAS instances (the reference's top-level builder with addBlas,
+ * BVHBuilder.cpp:288-301,1521-1533) uploads as a two-level scene: its depth is the top level's + 1 +
+ * the deepest BLAS's, and only any-hit rtq_dispatch_rays and culling accept it. */
 rtq_status rtq_scene_upload(rtq_device* dev, const void* blob, size_t bytes, const rtq_ranges* ranges,
                             const float* positions_xyz, uint32_t num_vertices, rtq_scene** out);
 rtq_status rtq_scene_destroy(rtq_scene* scene);
@@ -172,7 +175,11 @@ rtq_status rtq_scene_get_info(const rtq_scene* scene, rtq_scene_info* info);
 /* dispatch-rays: traverseBvh (closest hit, Shaders/bvh/bvhTraversal_inline.glsl:3-78, via rayTrace,
  * baseRaytracingPass.glsl:16-27) or traverseBvhFast (any hit, bvhTraversal_inline.glsl:80-151).
  * d_leaf_nid (closest hit only, may be NULL) receives the reference node id of the leaf that held
- * the accepted triangle (ClosestHit_storeNid, bvhBindings.glsl:127-134), 0xFFFFFFFF on a miss. */
+ * the accepted triangle (ClosestHit_storeNid, bvhBindings.glsl:127-134), 0xFFFFFFFF on a miss.
+ * On a two-level scene RTQ_TRACE_ANY runs traverseTlasFast with tlas_collide_fast
+ * (bvhTraversal_inline.glsl:80-151, bvhCollision.glsl:321-349: a BLAS is entered when its header box
+ * test is > 0, the top level's own triangles are not tested); RTQ_TRACE_CLOSEST returns
+ * RTQ_ERR_INVALID_ARGUMENT, since the reference's closest-hit top-level walk calls an undefined HitBox. */
 rtq_status rtq_dispatch_rays(rtq_scene* scene, const rtq_ray* d_rays, uint32_t num_rays, uint32_t flags,
                              void* d_out, uint32_t* d_leaf_nid, void* stream);
 

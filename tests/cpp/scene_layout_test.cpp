@@ -92,7 +92,9 @@ void checkScene(const Built& s) {
         continue;
       }
       CHECK(it.ref != rtq::kRefEmpty);
-      const uint32_t first = it.ref & ~rtq::kRefLeaf;
+      const uint32_t first = it.ref & rtq::kLeafSlotMask;
+      CHECK(((it.ref >> rtq::kLeafCountShift) & 15u) == std::min(count, 16u) - 1u);  // leaf size in the ref
+      CHECK((it.ref & rtq::kRefInst) == 0);
       for (uint32_t k = 0; k < count; ++k) {
         CHECK(first + k < h.tris.size());
         CHECK(usedSlots.insert(first + k).second);
@@ -368,8 +370,9 @@ void checkTwoLevel(const Built& s, uint32_t nblas) {
       CHECK(it.ref == rtq::kRefEmpty);
       continue;
     }
-    const uint32_t first = it.ref & ~rtq::kRefLeaf;
+    const uint32_t first = it.ref & rtq::kLeafSlotMask;
     CHECK((it.ref & rtq::kRefLeaf) != 0 && (it.ref & rtq::kRefInst) == 0);
+    CHECK(((it.ref >> rtq::kLeafCountShift) & 15u) == std::min(ntri, 16u) - 1u);
     for (uint32_t k = 0; k < ntri; ++k) {
       CHECK(first + k < h.tris.size());
       uint32_t w[4];

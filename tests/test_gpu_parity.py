@@ -150,6 +150,68 @@ def test_dispatch_modes_identical(persistent, blocks, sort):
         d.set_option(B.RTQ_OPTION_SORT_RAYS, keep[2])
 
 
+def _leaf_sizes(blob, ranges):
+    """Triangle count of every leaf of a one-level blob (the packed count word's low 12 bits)."""
+    nodes = np.frombuffer(blob, np.uint32, ranges.nodes.size // 4, ranges.nodes.offset).reshape(-1, 16)
+    leaf = np.frombuffer(blob, np.uint32, ranges.leaf_data.size // 4, ranges.leaf_data.offset)
+    is_leaf = (nodes[:, 2] & 0x3FFFFFFF) == 0x3FFFFFFF
+    off = nodes[is_leaf, 3]
+    off = off[off != 0xFFFFFFFF]
+    return leaf[off] & 0xFFF
+
+
+def test_large_leaves_every_dispatch():
+    """Leaves of 7 to 20 triangles (spheres and a soup built with a depth limit of 12 and 12 items per
+    leaf; the other tests' scenes hold the small leaves). A leaf ref carries min(n, 16) - 1, and the
+    lane-refilling kernels issue the loads of up to four slots at once, so every leaf size, those past
+    16 slots included, must give the oracle's hits, flags and counters in every dispatch mode and in the
+    batched form."""
+    _print("case: spheres + soup, 40240 tris, leaves of 7-20 tris (max depth 12, 12 per leaf), all dispatch modes")
+    b = H.BVHBuilder()
+    b.generate(B.RTQ_SCENE_SPHERES, 5, 40)
+    b.generate(B.RTQ_SCENE_SOUP, 11, 30_000)
+    b.set_parameters(max_depth=12, min_obj_per_node=12)
+    b.build(16)
+    blob, ranges = b.fill_gpu_buffer()
+    sizes = _leaf_sizes(blob, ranges)
+    assert (sizes > 16).sum() >= 20 and (sizes <= 16).sum() > 1000
+    d = G.dev()
+    gs = D.Scene(d, blob, ranges, b.positions())
+    os_ = oracle.Scene(blob, ranges, b.positions())
+    rays = b.surface_rays(seed=12, count=60_000)
+    o = os_.trace_closest(rays, threads=16)
+    any_rays = rays.copy()
+    any_rays["tmax"] = np.float32(30.0)
+    oa = os_.trace_any(any_rays, threads=16)[0]
+    assert 0.05 < oa.mean() < 0.95
+    g = gs.trace_closest(rays, counted=True)
+    G.assert_closest_equal(g, o, "large leaves counted")
+    assert np.array_equal(g["counters"], o["counters"])
+    keep = d.get_option(B.RTQ_OPTION_PERSISTENT)
+    try:
+        for persistent in (0, 1, 2):
+            d.set_option(B.RTQ_OPTION_PERSISTENT, persistent)
+            G.assert_closest_equal(gs.trace_closest(rays), o, f"large leaves persistent={persistent}")
+            assert np.array_equal(gs.trace_any(any_rays), oa)
+    finally:
+        d.set_option(B.RTQ_OPTION_PERSISTENT, keep)
+    # the batched form, lane refill (chunk 1) and chunks, closest and any hit
+    n = rays.size
+    d_r, d_a = d.upload(rays), d.upload(any_rays)
+    d_h, d_n, d_o = d.alloc(16 * n), d.alloc(4 * n), d.alloc(4 * n)
+    batch = D.Batch(d, [gs], [(0, B.RTQ_TRACE_CLOSEST, d_r.ptr, n, d_h.ptr, d_n.ptr),
+                              (0, B.RTQ_TRACE_ANY, d_a.ptr, n, d_o.ptr, None)])
+    for chunk in (1, 128):
+        batch.set_params(chunk_rays=chunk, blocks_per_cu=8, stack_depth=32, use_graph=0, leaf_wait=0)
+        d_h.write(np.zeros(16 * n, np.uint8))
+        d_o.write(np.zeros(4 * n, np.uint8))
+        batch.launch()
+        h = d_h.read(B.HIT_DTYPE, n)
+        G.assert_closest_equal({"tri_id": h["tri_id"], "t": h["t"], "u": h["u"], "v": h["v"],
+                                "nid": d_n.read(np.uint32, n)}, o, f"large leaves batch chunk={chunk}")
+        assert np.array_equal(d_o.read(np.uint32, n), oa)
+
+
 def test_sorted_dispatches_on_two_streams():
     """Sorting dispatches share the device's sort workspace; issued back to back on two streams with no
     host synchronisation in between (the second one growing the workspace, on a fresh device), each must

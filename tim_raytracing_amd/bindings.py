@@ -14,7 +14,8 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 # RTQ_BUILD_LIB names another build of the host library (the tests' sanitizer builds)
 BUILD_LIB = os.environ.get("RTQ_BUILD_LIB") or os.path.join(PKG_DIR, "librtq_build.so")
-DEVICE_LIB = os.path.join(PKG_DIR, "librtq.so")
+# RTQ_DEVICE_LIB names another build of the device library (a build with other compile-time settings)
+DEVICE_LIB = os.environ.get("RTQ_DEVICE_LIB") or os.path.join(PKG_DIR, "librtq.so")
 
 RTQ_OK = 0
 STATUS_NAMES = {

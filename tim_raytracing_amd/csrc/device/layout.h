@@ -7,8 +7,10 @@
 //   b = (b0.max.y, b0.max.z, b1.min.x, b1.min.y)
 //   c = (b1.min.z, b1.max.x, b1.max.y, b1.max.z)
 //   d = (ref child0, ref child1, reference node id, 0)
-// A child ref is an inner index (< kRefDone), kRefLeaf | first triangle slot of a leaf, or kRefEmpty
-// for a leaf holding no triangle. Leaves need no record: visiting one reads its triangle slots.
+// A child ref is an inner index (< kRefDone), a leaf ref kRefLeaf | (min(n, 16) - 1) << 26 | first
+// triangle slot for a leaf of n triangles, or kRefEmpty for a leaf holding no triangle. Leaves need no
+// record: visiting one reads its triangle slots, and the count lets a walk issue the loads of several
+// slots at once (the last-slot flag still ends the leaf).
 //
 // Triangle slots: one 48 B record per inline triangle of every leaf, leaves in the same depth-first
 // order, triangles in blob (sorted leaf) order:
@@ -36,6 +38,11 @@ constexpr uint32_t kRefDone = 0x7FFFFFFFu;  // traversal finished (never a valid
 // 2^30, so bit 30 tells the two kinds apart.
 constexpr uint32_t kRefInst = 0x40000000u;
 constexpr uint32_t kRefIndex = 0x3FFFFFFFu;
+// Triangle leaf refs: bits 26-29 hold min(triangles in the leaf, 16) - 1, bits 0-25 the first slot
+// (a scene holds fewer than 2^26 slots).
+constexpr uint32_t kLeafCountShift = 26;
+constexpr uint32_t kLeafSlotMask = (1u << kLeafCountShift) - 1u;
+constexpr uint32_t kMaxSlots = 1u << kLeafCountShift;
 
 struct alignas(16) DevNode {
   float a[4], b[4], c[4];

@@ -48,8 +48,8 @@ struct TriKeyHash {
 
 using TriIndexMap = std::unordered_map<TriKey, uint32_t, TriKeyHash>;
 
-// The triangle slots of blob leaf `id` appended to h.tris (layout.h); *ref receives kRefLeaf | first
-// slot, or stays kRefEmpty for a leaf without triangles.
+// The triangle slots of blob leaf `id` appended to h.tris (layout.h); *ref receives the leaf ref
+// (first slot and count), or stays kRefEmpty for a leaf without triangles.
 inline rtq_status appendLeafSlots(const uint32_t* nodes, const uint32_t* leaf, uint32_t leafWords,
                                   const float* positions, uint32_t numVertices, const TriIndexMap& triIndex,
                                   uint32_t id, HostScene& h, uint32_t& ref, const char** err) {
@@ -63,7 +63,8 @@ inline rtq_status appendLeafSlots(const uint32_t* nodes, const uint32_t* leaf, u
   const uint32_t count = leaf[off] & 0xFFFu;
   if (count == 0) return RTQ_OK;
   if (size_t(off) + 1 + 3 * size_t(count) > leafWords) return fail(RTQ_ERR_BAD_BLOB, "leaf triangles overrun");
-  ref = kRefLeaf | uint32_t(h.tris.size());
+  if (h.tris.size() + count > kMaxSlots) return fail(RTQ_ERR_LIMIT, "more than 2^26 triangle slots in a scene");
+  ref = kRefLeaf | ((std::min<uint32_t>(count, 16u) - 1u) << kLeafCountShift) | uint32_t(h.tris.size());
   for (uint32_t k = 0; k < count; ++k) {
     const uint32_t* t = leaf + off + 1 + 3 * k;
     const uint32_t vi[3] = {t[0] + (t[1] & 0xFFFFu), t[0] + (t[1] >> 16), t[0] + (t[2] & 0xFFFFu)};
@@ -254,7 +255,6 @@ inline rtq_status buildTwoLevel(const uint8_t* blob, size_t bytes, const rtq_ran
   // current leaf's instances, and the BLAS walk's deferred children
   h.maxDepth = topDepth + 1 + blasDepth;
   if (h.maxDepth > 64) return fail(RTQ_ERR_LIMIT, "top-level plus BLAS depth above 63");
-  if (h.tris.size() > kRefIndex) return fail(RTQ_ERR_LIMIT, "more than 2^30 triangle slots in a two-level scene");
   h.numInner = uint32_t(h.nodes.size());
   boxFlags(h);
   for (const DevInst& in : h.insts)

@@ -840,6 +840,69 @@ def bench_serve(args):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     plain = run(True, arrivals)
     served = run(False, arrivals)
+
+    # The same requests with rays and results in device memory (rays made and results consumed by other
+    # GPU passes): rtq_server_submit_device traces them in place, so nothing crosses the host link and
+    # both forms are bound by their kernels and launches. Same order of runs; the at-load served
+    # results must equal the host-buffer run's on every ray.
+    d_pools = [dev.upload(pools[i]) for i in range(distinct)]
+    out_bytes = np.where(req_any, 4, 16) * req_sizes
+    dout_off = np.concatenate([[0], np.cumsum((out_bytes + 255) // 256 * 256)]).astype(np.int64)  # 256 B aligned
+    d_arena = dev.alloc(int(dout_off[-1]))
+
+    def run_dev(plain_form, arrivals_dev):
+        srv = D.Server(dev, scenes, budget_bytes=budget, plain=plain_form, max_batch_rays=args.max_batch_rays,
+                       params=params, max_inflight=args.max_inflight)
+        handles = [None] * nreq
+        start = time.perf_counter()
+        for k in range(nreq):
+            if arrivals_dev is not None:
+                wait = start + arrivals_dev[k] - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+            si = int(req_scene[k])
+            n = int(req_sizes[k])
+            rays = D.DeviceView(d_pools[si % distinct], 32 * req_off[k], 32 * n)
+            flags = B.RTQ_TRACE_ANY if req_any[k] else B.RTQ_TRACE_CLOSEST
+            out = D.DeviceView(d_arena, int(dout_off[k]), int(out_bytes[k]))
+            handles[k] = srv.submit_device(si, flags, rays, n, out)
+        lat = []
+        for h in handles:
+            assert h.wait() == B.RTQ_OK
+            lat.append(h.stats().latency_ms)
+        end = time.perf_counter()
+        batches = len({h.stats().batch_id for h in handles})
+        for h in handles:
+            h.release()
+        srv.close()
+        lat = np.array(lat)
+        return {"mrays_s": round(float(req_sizes.sum()) / (end - start) / 1e6, 2),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "makespan_s": round(end - start, 3), "launches": batches}
+
+    run_dev(False, None)
+    dplain_cap = run_dev(True, None)
+    dserved_cap = run_dev(False, None)
+    drate = 1.5 * dplain_cap["mrays_s"] * 1e6 / float(req_sizes.mean())
+    darrivals = np.cumsum(np.random.default_rng(7).exponential(1.0 / drate, nreq))
+    dplain = run_dev(True, darrivals)
+    dserved = run_dev(False, darrivals)
+    dres = d_arena.read(np.uint8, int(dout_off[-1]))
+    hres = out_arena.view(np.uint8, int(out_off[-1]))
+    same = all(np.array_equal(dres[dout_off[k]:dout_off[k] + out_bytes[k]], hres[out_off[k]:out_off[k] + out_bytes[k]])
+               for k in range(nreq))
+    del dres
+    device_requests = {
+        "served": dserved, "plain": dplain,
+        "speedup_vs_plain": round(dserved["mrays_s"] / dplain["mrays_s"], 3) if dplain["mrays_s"] else None,
+        "capacity": {"served": dserved_cap, "plain": dplain_cap,
+                     "speedup_vs_plain": round(dserved_cap["mrays_s"] / dplain_cap["mrays_s"], 3)},
+        "arrival_rate_per_s": round(drate, 2),
+        "results_equal_host_requests": same,
+        "requests": "the same 1000 requests, rays and results in HBM (rtq_server_submit_device); Poisson arrivals "
+                    "at 1.5x the plain form's capacity on such requests",
+    }
+    del d_pools, d_arena
     checks = {}
     if not args.no_cpu_baseline:
         import oracle  # the CPU baseline leg is the only place bench.py touches the oracle
@@ -884,6 +947,7 @@ def bench_serve(args):
         "speedup_vs_plain": round(served["mrays_s"] / plain["mrays_s"], 3) if plain["mrays_s"] else None,
         "capacity": {"served": served_cap, "plain": plain_cap,
                      "speedup_vs_plain": round(served_cap["mrays_s"] / plain_cap["mrays_s"], 3)},
+        "device_requests": device_requests,
         # over the saturated served run's makespan: the VALU instructions of all its launches (committed
         # counters of one such run) per second of wall time, a lower bound of the kernels' own issue rate
         "roofline": roofline("serve-64-scenes-poisson", f"{kname}@saturated-stream",

@@ -330,7 +330,7 @@ typedef struct rtq_server_config {
 
 typedef struct rtq_request_stats {
   double queued_ms;   /* submit -> admitted */
-  double latency_ms;  /* submit -> results in host memory */
+  double latency_ms;  /* submit -> results in host memory (device requests: in device memory) */
   uint32_t batch_id;  /* launch that carried the request */
   uint32_t batch_requests;
 } rtq_request_stats;
@@ -345,6 +345,14 @@ rtq_status rtq_server_destroy(rtq_server* srv);
  * must stay valid until the request completes. RTQ_ERR_BUDGET: the request alone exceeds the budget. */
 rtq_status rtq_server_submit(rtq_server* srv, uint32_t scene, uint32_t flags, const rtq_ray* host_rays,
                              uint32_t num_rays, void* host_out, uint32_t* host_leaf_nid, rtq_request** req);
+/* The same for a request whose rays and results live in device memory on the server's device (rays
+ * made and results consumed by other GPU passes, as the reference's passes hand their buffers to
+ * each other: cameraPass.comp writes the record buffer the next pass reads, raytracingPass.cpp:112-137).
+ * Traced in place: no arena bytes (the budget does not apply), no copies over the host link; the
+ * request completes when its results are in d_out / d_leaf_nid. d_rays and closest-hit d_out must be
+ * 16-byte aligned, any-hit d_out and d_leaf_nid 4-byte aligned; all must stay valid until completion. */
+rtq_status rtq_server_submit_device(rtq_server* srv, uint32_t scene, uint32_t flags, const rtq_ray* d_rays,
+                                    uint32_t num_rays, void* d_out, uint32_t* d_leaf_nid, rtq_request** req);
 /* Blocks up to timeout_ms (0xFFFFFFFF = forever); returns the request's final status, or
  * RTQ_ERR_TIMEOUT if it is still running. */
 rtq_status rtq_request_wait(rtq_request* req, uint32_t timeout_ms);

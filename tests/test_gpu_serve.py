@@ -101,6 +101,88 @@ def test_serve_mix_equals_alone():
     srv.close()
 
 
+def test_serve_device_requests_equal_alone():
+    """Requests whose rays and results live in device memory (rtq_server_submit_device) are traced in
+    place: mixed with host-buffer requests, batched, from several threads, each equals the request
+    traced alone and the oracle; misaligned and undersized device buffers are rejected."""
+    print("case: 8 resident scenes, 40 device-resident requests + 10 host requests, 4 threads, vs alone", flush=True)
+    dev = G.dev()
+    builders = _scenes(8, 17)
+    scenes = [D.Scene.from_builder(dev, b) for b in builders]
+    reqs = _requests(builders, 50, 18)
+    orc = [oracle.Scene(*b.fill_gpu_buffer(), b.positions()) for b in builders]
+    alone = []
+    for si, flags, rays in reqs:
+        if flags == B.RTQ_TRACE_CLOSEST:
+            g = scenes[si].trace_closest(rays)
+            G.assert_closest_equal(g, orc[si].trace_closest(rays, threads=16), "device request alone vs oracle")
+            alone.append((g["hits"].view(np.uint32).copy(), g["nid"]))
+        else:
+            occ = scenes[si].trace_any(rays)
+            assert np.array_equal(occ, orc[si].trace_any(rays, threads=16)[0])
+            alone.append((occ, None))
+    bufs = []
+    for si, flags, rays in reqs:
+        n = rays.size
+        closest = flags == B.RTQ_TRACE_CLOSEST
+        bufs.append((dev.upload(rays), dev.alloc((16 if closest else 4) * n), dev.alloc(4 * n) if closest else None))
+    for params in (None, B.BatchParams(64, 4, 64, 0, 0)):
+        srv = D.Server(dev, scenes, budget_bytes=4 << 20, params=params)
+        handles = [None] * len(reqs)
+
+        def client(t):
+            for k in range(t, len(reqs), 4):
+                si, flags, rays = reqs[k]
+                if k % 5 == 4:   # every fifth request through host buffers, in the same batches
+                    handles[k] = srv.submit(si, flags, rays)
+                else:
+                    d_r, d_o, d_n = bufs[k]
+                    d_o.write(np.zeros(d_o.nbytes, np.uint8))
+                    handles[k] = srv.submit_device(si, flags, d_r, rays.size, d_o, d_n)
+
+        th = [threading.Thread(target=client, args=(t,)) for t in range(4)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        batches = set()
+        for k, h in enumerate(handles):
+            assert h.wait() == B.RTQ_OK
+            batches.add(h.stats().batch_id)
+            si, flags, rays = reqs[k]
+            a_out, a_nid = alone[k]
+            if k % 5 == 4:
+                got = h.out.view(np.uint32) if flags == B.RTQ_TRACE_CLOSEST else h.out
+                assert np.array_equal(got, a_out)
+            elif flags == B.RTQ_TRACE_CLOSEST:
+                _, d_o, d_n = bufs[k]
+                assert np.array_equal(d_o.read(np.uint32, 4 * rays.size), a_out)
+                assert np.array_equal(d_n.read(np.uint32, rays.size), a_nid)
+            else:
+                assert np.array_equal(bufs[k][1].read(np.uint32, rays.size), a_out)
+            h.release()
+        assert len(batches) < len(reqs)
+        srv.close()
+    # plain comparison mode on device requests
+    srv = D.Server(dev, scenes, budget_bytes=4 << 20, plain=True)
+    for k in range(6):
+        si, flags, rays = reqs[k]
+        d_r, d_o, d_n = bufs[k]
+        d_o.write(np.zeros(d_o.nbytes, np.uint8))
+        h = srv.submit_device(si, flags, d_r, rays.size, d_o, d_n)
+        assert h.wait() == B.RTQ_OK
+        got = d_o.read(np.uint32, (4 if flags == B.RTQ_TRACE_CLOSEST else 1) * rays.size)
+        assert np.array_equal(got, alone[k][0])
+        h.release()
+    # misaligned rays and a too-small output are rejected before anything is queued
+    d_r, d_o, _ = bufs[1]
+    with pytest.raises(B.RtqError):
+        srv.submit_device(0, B.RTQ_TRACE_ANY, D.DeviceView(d_r, 8, 32 * 10), 10, d_o)
+    with pytest.raises(ValueError):
+        srv.submit_device(0, B.RTQ_TRACE_CLOSEST, d_r, reqs[1][2].size, D.DeviceView(d_o, 0, 16))
+    srv.close()
+
+
 def test_serve_admission_limits():
     print("case: server budget rejection, queue-full, cancellation", flush=True)
     dev = G.dev()

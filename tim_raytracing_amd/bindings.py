@@ -180,6 +180,7 @@ _DEVICE_SIGS = {
     "rtq_server_create": ([P, P, U32, C.POINTER(ServerConfig), C.POINTER(P)], C.c_int),
     "rtq_server_destroy": ([P], C.c_int),
     "rtq_server_submit": ([P, U32, U32, P, U32, P, P, C.POINTER(P)], C.c_int),
+    "rtq_server_submit_device": ([P, U32, U32, P, U32, P, P, C.POINTER(P)], C.c_int),
     "rtq_request_wait": ([P, U32], C.c_int),
     "rtq_request_cancel": ([P], C.c_int),
     "rtq_request_get_stats": ([P, C.POINTER(RequestStats)], C.c_int),

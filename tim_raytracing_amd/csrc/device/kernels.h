@@ -235,10 +235,48 @@ __device__ __forceinline__ DevScene stageTop(const DevScene& s) {
   return v;
 }
 
-// One round of descent for the calling lanes. FASTONLY: every calling lane takes the reciprocal slab
-// form (decided per wave), so no per-lane branch is needed; otherwise `fast` chooses per lane.
-// OCT >= 0 (with FASTONLY): every lane's direction lies in octant OCT and the scene's boxes are
-// ordered, so boxHitOrd replaces boxHitFast.
+// One descent step at inner node `node` whose record is (a, b, c, d): both child boxes are tested
+// against lim, then the walk goes to the nearer of two hit children (deferring the other on the
+// stack), to the only hit child, or to the stack top. FASTONLY: the reciprocal slab form for every
+// calling lane (decided per wave); otherwise `fast` chooses per lane. OCT >= 0 (with FASTONLY): every
+// lane's direction lies in octant OCT and the scene's boxes are ordered, so boxHitOrd replaces
+// boxHitFast.
+template <int STACK, bool FASTONLY, int OCT>
+__device__ __forceinline__ void nodeStep(const float4 a, const float4 b, const float4 c, const uint4 d, uint32_t* stk,
+                                         const Ray3& r, const RayRcp& rq, bool fast, float lim, uint32_t& node,
+                                         int& sp) {
+  float d0, d1;
+  if constexpr (OCT >= 0) {
+    d0 = boxHitOrd<OCT>(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+    d1 = boxHitOrd<OCT>(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+  } else if (FASTONLY || fast) {
+    d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+    d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+  } else {
+    d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, lim);
+    d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, lim);
+  }
+  const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
+  if (h0 && h1) {
+    const bool n1 = d1 < d0;
+    if (sp < STACK) {
+      stk[sp * kTraceBlock] = n1 ? d.x : d.y;
+      ++sp;
+      node = n1 ? d.y : d.x;
+    } else {
+      // a deeper tree than the stack holds: the lane stops here instead of writing past its
+      // stack column (unreachable, every launch sizes STACK to the tree depth, stack_class.h)
+      node = kRefDone;
+      sp = 0;
+    }
+  } else if (h0 || h1) {
+    node = h0 ? d.x : d.y;
+  } else {
+    node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
+  }
+}
+
+// One round of descent for the calling lanes (see nodeStep for FASTONLY and OCT).
 template <int STACK, bool FASTONLY, bool COUNT, int OCT = -1>
 __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, const DevNode* top, uint32_t topSlots,
                                              uint32_t* stk, const Ray3& r, const RayRcp& rq, bool fast, float lim,
@@ -259,35 +297,42 @@ __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, 
       const float4 a = np[0], b = np[1], c = np[2];
       const uint4 d = reinterpret_cast<const uint4*>(np)[3];
       if (COUNT) ++inner;
-      float d0, d1;
-      if constexpr (OCT >= 0) {
-        d0 = boxHitOrd<OCT>(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
-        d1 = boxHitOrd<OCT>(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
-      } else if (FASTONLY || fast) {
-        d0 = boxHitFast(r, rq, a.x, a.y, a.z, a.w, b.x, b.y, lim);
-        d1 = boxHitFast(r, rq, b.z, b.w, c.x, c.y, c.z, c.w, lim);
-      } else {
-        d0 = boxHit(r, a.x, a.y, a.z, a.w, b.x, b.y, lim);
-        d1 = boxHit(r, b.z, b.w, c.x, c.y, c.z, c.w, lim);
-      }
-      const bool h0 = d0 >= 0.0f, h1 = d1 >= 0.0f;
-      if (h0 && h1) {
-        const bool n1 = d1 < d0;
-        if (sp < STACK) {
-          stk[sp * kTraceBlock] = n1 ? d.x : d.y;
-          ++sp;
-          node = n1 ? d.y : d.x;
-        } else {
-          // a deeper tree than the stack holds: the lane stops here instead of writing past its
-          // stack column (unreachable, every launch sizes STACK to the tree depth, stack_class.h)
-          node = kRefDone;
-          sp = 0;
-        }
-      } else if (h0 || h1) {
-        node = h0 ? d.x : d.y;
-      } else {
-        node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
-      }
+      nodeStep<STACK, FASTONLY, OCT>(a, b, c, d, stk, r, rq, fast, lim, node, sp);
+    }
+  }
+}
+
+// Closest-hit walks of the lane-refilling kernels hold the record of the lane's current inner node in
+// registers across steps, rounds and leaf phases: its loads are issued as soon as the node is known
+// (after a step, when a ray starts, or when a leaf is reached: the stack top is then the node that
+// follows the leaf, since a closest-hit walk always goes on), so the record of the node popped after a
+// leaf arrives while the leaf is tested (soup bounces: +2 %).
+struct NodeRec {
+  float4 a, b, c;
+  uint4 d;
+};
+
+__device__ __forceinline__ void loadRec(const DevNode* __restrict__ nodes, uint32_t node, NodeRec& q) {
+  const float4* np = reinterpret_cast<const float4*>(nodes + node);
+  q.a = np[0];
+  q.b = np[1];
+  q.c = np[2];
+  q.d = reinterpret_cast<const uint4*>(np)[3];
+}
+
+template <int STACK, bool FASTONLY>
+__device__ __forceinline__ void descendRoundHeld(const DevNode* __restrict__ nodes, NodeRec& q, uint32_t* stk,
+                                                 const Ray3& r, const RayRcp& rq, bool fast, float lim,
+                                                 uint32_t& node, int& sp, uint32_t wait, uint32_t extra) {
+  const uint32_t live = uint32_t(__popcll(__ballot(1)));
+  const uint32_t need = wait ? wait : 1u;
+  for (;;) {
+    const bool down = node < kRefDone;
+    const uint64_t m = __ballot(down);
+    if (m == 0 || live - uint32_t(__popcll(m)) + extra >= need) break;
+    if (down) {
+      nodeStep<STACK, FASTONLY, -1>(q.a, q.b, q.c, q.d, stk, r, rq, fast, lim, node, sp);
+      if (node < kRefDone) loadRec(nodes, node, q);
     }
   }
 }
@@ -499,6 +544,7 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
   Ray3 r{};
   RayRcp rq{};
   Closest hit{};
+  NodeRec rec{};
   for (;;) {
     const uint64_t idle = __ballot(!active);
     if (idle && !exhausted) {
@@ -518,6 +564,7 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
             hit = Closest{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};
             node = s.root;
             sp = 0;
+            if (!ANY && node < kRefDone) loadRec(s.nodes, node, rec);
             active = true;
           }
         }
@@ -529,17 +576,36 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
     if (!active) continue;
     bool accepted = false;
     const float lim = ANY ? tmax : hit.t;
-    uint32_t unused = 0;
-    if (__all(fast ? 1 : 0))
-      descendRound<STACK, true, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
-    else
-      descendRound<STACK, false, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
+    const bool allFast = __all(fast ? 1 : 0);
+    if constexpr (ANY) {
+      // any hit: a leaf may end the walk, so nothing is loaded ahead (measured: holding records is
+      // 1 % slower here)
+      uint32_t unused = 0;
+      if (allFast)
+        descendRound<STACK, true, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
+      else
+        descendRound<STACK, false, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
+    } else {
+      if (allFast)
+        descendRoundHeld<STACK, true>(s.nodes, rec, stk, r, rq, fast, lim, node, sp, wait, extra);
+      else
+        descendRoundHeld<STACK, false>(s.nodes, rec, stk, r, rq, fast, lim, node, sp, wait, extra);
+    }
     if (ANY && s.twoLevel && node > kRefDone && node != kRefEmpty && (node & kRefInst)) {
       node = enterInstance<STACK>(s, stk, sp, node, r, rq, fast, tmax);
     } else if (node > kRefDone) {
+      // closest hit: the node after this leaf is the stack top (the walk always goes on), and its record
+      // is loaded while the leaf is tested
+      const uint32_t top = sp > 0 ? stk[(sp - 1) * kTraceBlock] : kRefDone;
+      if (!ANY && top < kRefDone) loadRec(s.nodes, top, rec);
       uint32_t unusedTests = 0;
       if (node != kRefEmpty) accepted = leafTests<kRefillLeafGroup, false>(s.tris, node, r, ANY, tmax, hit, unusedTests);
-      node = (!accepted && sp > 0) ? stk[(--sp) * kTraceBlock] : kRefDone;
+      if (!accepted && sp > 0) {
+        node = top;
+        --sp;
+      } else {
+        node = kRefDone;
+      }
     }
     if (node == kRefDone) {
       if (ANY) {
@@ -952,6 +1018,7 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
   Ray3 r{};
   RayRcp rq{};
   Closest hit{};
+  NodeRec rec{};
   for (;;) {
     const uint64_t idle = __ballot(!active);
     if (idle && !exhausted && next >= end && !takeChunk(counter, totalRays, kRefillChunk, q, tried, next, end))
@@ -982,6 +1049,7 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
           hit = Closest{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};  // any hit keeps t = tmax throughout
           node = s.root;
           sp = 0;
+          if (node < kRefDone) loadRec(nodes, node, rec);
           active = true;
         }
       }
@@ -990,18 +1058,26 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
     const uint32_t extra = exhausted ? 0u : uint32_t(__popcll(__ballot(!active)));
     if (!active) continue;
     bool accepted = false;
-    uint32_t unused = 0;
     // lanes of one block trace different scenes here, so nothing is staged in LDS
     if (__all(fast ? 1 : 0))
-      descendRound<STACK, true, false>(nodes, nullptr, 0u, stk, r, rq, fast, hit.t, node, sp, wait, extra, unused);
+      descendRoundHeld<STACK, true>(nodes, rec, stk, r, rq, fast, hit.t, node, sp, wait, extra);
     else
-      descendRound<STACK, false, false>(nodes, nullptr, 0u, stk, r, rq, fast, hit.t, node, sp, wait, extra, unused);
+      descendRoundHeld<STACK, false>(nodes, rec, stk, r, rq, fast, hit.t, node, sp, wait, extra);
     if (node > kRefDone) {
+      // a closest-hit walk goes on with the stack top: its record is loaded while the leaf is tested
+      const uint32_t top = sp > 0 ? stk[(sp - 1) * kTraceBlock] : kRefDone;
+      if (!anyRay && top < kRefDone) loadRec(nodes, top, rec);
       uint32_t unusedTests = 0;
       // any hit keeps hit.t = tmax, so the closest-hit bound serves both modes
       if (node != kRefEmpty)
         accepted = leafTests<kRefillLeafGroup, false>(tris, node, r, anyRay, hit.t, hit, unusedTests);
-      node = (!accepted && sp > 0) ? stk[(--sp) * kTraceBlock] : kRefDone;
+      if (!accepted && sp > 0) {
+        node = top;
+        --sp;
+        if (anyRay && node < kRefDone) loadRec(nodes, node, rec);
+      } else {
+        node = kRefDone;
+      }
     }
     if (node == kRefDone) {
       const BatchItemDev& it = items[item];

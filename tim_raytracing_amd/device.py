@@ -47,6 +47,20 @@ class DeviceBuffer:
         return out
 
 
+class DeviceView(DeviceBuffer):
+    """A byte range of a DeviceBuffer; keeps the buffer alive and frees nothing itself."""
+
+    def __init__(self, parent: DeviceBuffer, offset: int, nbytes: int):
+        assert 0 <= offset and offset + nbytes <= parent.nbytes
+        self.dev = parent.dev
+        self.parent = parent
+        self.nbytes = int(nbytes)
+        self.ptr = parent.ptr + int(offset)
+
+    def free(self):
+        self.ptr = None
+
+
 class Stream:
     """A HIP stream of the library's runtime (default priority)."""
 
@@ -311,6 +325,18 @@ class Server:
                        self.dev.lib.rtq_server_submit(self.handle, scene, flags, rays.ctypes.data, n, out.ctypes.data,
                                                       nid.ctypes.data if nid is not None else None, C.byref(h)))
         return Request(self, h, rays, out, nid)
+
+    def submit_device(self, scene: int, flags: int, d_rays, n: int, d_out, d_nid=None) -> Request:
+        """A request whose rays and results are device buffers (DeviceBuffer objects, kept alive by the
+        request): traced in place, nothing crosses the host link."""
+        need = 16 * n if flags == B.RTQ_TRACE_CLOSEST else 4 * n
+        if d_rays.nbytes < 32 * n or d_out.nbytes < need or (d_nid is not None and d_nid.nbytes < 4 * n):
+            raise ValueError("device buffers smaller than the request")
+        h = C.c_void_p()
+        B.check_device("rtq_server_submit_device",
+                       self.dev.lib.rtq_server_submit_device(self.handle, scene, flags, d_rays.ptr, n, d_out.ptr,
+                                                             d_nid.ptr if d_nid is not None else None, C.byref(h)))
+        return Request(self, h, d_rays, d_out, d_nid)
 
     def close(self):
         if getattr(self, "handle", None):

@@ -399,9 +399,12 @@ __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const
 }
 
 // Whether this ray may use boxHitFast (see quotient()); the scene flag covers the box bounds.
-__device__ __forceinline__ bool fastLane(const DevScene& s, const Ray3& r, float tmax) {
-  return s.fastBoxes && fastDirComponent(r.dx) && fastDirComponent(r.dy) && fastDirComponent(r.dz) &&
+__device__ __forceinline__ bool fastRay(uint32_t fastBoxes, const Ray3& r, float tmax) {
+  return fastBoxes && fastDirComponent(r.dx) && fastDirComponent(r.dy) && fastDirComponent(r.dz) &&
          fastCoord(r.ox) && fastCoord(r.oy) && fastCoord(r.oz) && tmax == tmax;
+}
+__device__ __forceinline__ bool fastLane(const DevScene& s, const Ray3& r, float tmax) {
+  return fastRay(s.fastBoxes, r, tmax);
 }
 
 // One traversal routine per wave: the fast slab division when every active lane qualifies, and its
@@ -938,9 +941,22 @@ struct BatchItemDev {
   void* out;
   uint32_t* nid;
   uint32_t rayBegin;  // rays of all earlier items
-  uint32_t pad;
+  // the scene fields a lane needs to start a ray (setItemScene), so that the lane-refilling kernel
+  // starts one from the item record alone instead of a further dependent load of the scene record
+  uint32_t root;
+  const DevNode* nodes;
+  const DevTri* tris;
+  uint32_t fastBoxes;
+  uint32_t pad[3];
 };
-static_assert(sizeof(BatchItemDev) == 48, "BatchItemDev is 48 B");
+static_assert(sizeof(BatchItemDev) == 80, "BatchItemDev is 80 B");
+
+inline void setItemScene(BatchItemDev& d, const DevScene& v) {
+  d.root = v.root;
+  d.nodes = v.nodes;
+  d.tris = v.tris;
+  d.fastBoxes = v.fastBoxes;
+}
 
 // Zero the kXcds chunk-queue words of a refilling launch (the chunked batch kernel uses the first).
 __global__ void kResetCounter(uint32_t* counter) {
@@ -1000,8 +1016,7 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* 
 // follows the reference visit order exactly; every wave reaches the exit test once every share is used
 // up.
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __restrict__ scenes,
-                                                            const BatchItemDev* __restrict__ items,
+__global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const BatchItemDev* __restrict__ items,
                                                             uint32_t numItems, uint32_t totalRays,
                                                             uint32_t* __restrict__ counter, uint32_t wait) {
   __shared__ uint32_t lds[STACK * kTraceBlock];
@@ -1027,27 +1042,32 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const DevScene* __re
       const uint32_t want = uint32_t(__popcll(idle));
       const uint32_t base = next;
       next = (end - next > want) ? next + want : end;
+      // The item holding `base` (the last one whose rayBegin is <= base), searched once by the wave: the
+      // rays it hands out are consecutive, so each lane steps forward from there past the few item
+      // boundaries they may cross, instead of a per-lane search of dependent loads before every ray.
+      uint32_t first = 0, hi = numItems;
+      while (hi - first > 1u) {
+        const uint32_t mid = (first + hi) >> 1;
+        if (items[mid].rayBegin <= base) first = mid;
+        else hi = mid;
+      }
+      first = __builtin_amdgcn_readfirstlane(first);
       if (!active) {
         const uint32_t k = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
         if (k < next) {
-          uint32_t lo = 0, hi = numItems;
-          while (hi - lo > 1u) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (items[mid].rayBegin <= k) lo = mid;
-            else hi = mid;
-          }
+          uint32_t lo = first;
+          while (lo + 1u < numItems && items[lo + 1u].rayBegin <= k) ++lo;
           item = lo;
           const BatchItemDev& it = items[lo];
-          const DevScene& s = scenes[it.scene];
           rid = k - it.rayBegin;
-          nodes = s.nodes;
-          tris = s.tris;
+          nodes = it.nodes;
+          tris = it.tris;
           r = loadRay(it.rays, rid, tmax);
-          fast = fastLane(s, r, tmax);
+          fast = fastRay(it.fastBoxes, r, tmax);
           if (fast) rq = RayRcp{recip64(r.dx), recip64(r.dy), recip64(r.dz)};
           anyRay = it.flags != 1u;
           hit = Closest{tmax, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f, 0.0f};  // any hit keeps t = tmax throughout
-          node = s.root;
+          node = it.root;
           sp = 0;
           if (node < kRefDone) loadRec(nodes, node, rec);
           active = true;

@@ -341,33 +341,69 @@ __device__ __forceinline__ void descendRoundHeld(const DevNode* __restrict__ nod
 // The top-level tree and every BLAS are inner-node records of one array, walked by the same descent.
 // At a top-level leaf, tlas_collide_fast (bvhCollision.glsl:321-349) takes each listed instance whose
 // BlasHeader box CollideBox reports strictly positive (> 0: a ray starting inside the box gets 0 and
-// skips that BLAS) and walks the BLAS from its root; the leaf's own triangles are never tested. Here a
-// lane reaching an instance ref (kRefLeaf | kRefInst | k) scans the leaf's instances from k for the
-// next accepted one, leaves a resume ref for the instances after it on its stack, and continues with
-// that BLAS's root; the walk of the BLAS pops back to the resume ref when it is done. An any-hit
-// answer (is some triangle reachable this way accepted?) does not depend on the visit order. The
-// slab division takes the reciprocal form when the lane's ray and the scene qualify (the header boxes
-// are part of that check), with the same results as the IEEE division.
+// skips that BLAS) and walks the BLAS from its root; the leaf's own triangles are never tested. Here an
+// instance ref (kRefLeaf | kRefInst | k, the leaf's instances from k on) is a step of the descent like
+// an inner node: the step tests instance k's header box; if it accepts, the walk leaves a resume ref for
+// the instances after k on its stack (unless k is the leaf's last) and goes on with that BLAS's root,
+// whose walk pops back to the resume ref when it is done; if not, it goes on with instance k + 1, or
+// the stack top after the last one. Lanes at instances thus step with the lanes at nodes instead of
+// scanning a leaf's instance list in a loop of their own. An any-hit answer (is some triangle reachable
+// this way accepted?) does not depend on the visit order. The slab division takes the reciprocal form
+// when the lane's ray and the scene qualify (the header boxes are part of that check), with the same
+// results as the IEEE division.
+__device__ __forceinline__ bool isInstRef(uint32_t node) {
+  return node > kRefDone && node != kRefEmpty && (node & kRefInst) != 0;
+}
+
 template <int STACK>
-__device__ __forceinline__ uint32_t enterInstance(const DevScene& s, uint32_t* stk, int& sp, uint32_t node,
-                                                  const Ray3& r, const RayRcp& rq, bool fast, float tmax) {
-  for (uint32_t k = node & kRefIndex;; ++k) {
-    const float4* ip = reinterpret_cast<const float4*>(s.insts + k);
-    const float4 lo = ip[0], hi = ip[1];
-    const float t = fast ? boxHitFast(r, rq, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, tmax)
-                         : boxHit(r, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, tmax);
-    const bool last = __float_as_uint(hi.w) != 0u;
-    if (t > 0.0f) {
-      if (!last) {
-        if (sp >= STACK) return kRefDone;  // unreachable: STACK covers both levels and the resume entry
-        stk[sp * kTraceBlock] = kRefLeaf | kRefInst | (k + 1u);
-        ++sp;
+__device__ __forceinline__ void instanceStep(const DevInst* __restrict__ insts, uint32_t* stk, const Ray3& r,
+                                             const RayRcp& rq, bool fast, float tmax, uint32_t& node, int& sp) {
+  const uint32_t k = node & kRefIndex;
+  const float4* ip = reinterpret_cast<const float4*>(insts + k);
+  const float4 lo = ip[0], hi = ip[1];
+  const float t = fast ? boxHitFast(r, rq, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, tmax)
+                       : boxHit(r, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, tmax);
+  const bool last = __float_as_uint(hi.w) != 0u;
+  if (t > 0.0f) {
+    if (!last) {
+      if (sp >= STACK) {  // unreachable: STACK covers both levels and the resume entry
+        node = kRefDone;
+        sp = 0;
+        return;
       }
-      return __float_as_uint(lo.w);
+      stk[sp * kTraceBlock] = kRefLeaf | kRefInst | (k + 1u);
+      ++sp;
     }
-    if (last) break;
+    node = __float_as_uint(lo.w);
+  } else if (!last) {
+    node = kRefLeaf | kRefInst | (k + 1u);
+  } else {
+    node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
   }
-  return sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
+}
+
+// A round of descent over a two-level scene: lanes at inner nodes take a node step, lanes at instance
+// refs an instance step; a round ends as descendRound's does, with lanes at triangle leaves ready.
+template <int STACK, bool FASTONLY, int OCT = -1>
+__device__ __forceinline__ void descendRoundTwoLevel(const DevScene& s, uint32_t* stk, const Ray3& r,
+                                                     const RayRcp& rq, bool fast, float tmax, uint32_t& node, int& sp,
+                                                     uint32_t wait, uint32_t extra) {
+  const uint32_t live = uint32_t(__popcll(__ballot(1)));
+  const uint32_t need = wait ? wait : 1u;
+  for (;;) {
+    const bool inst = isInstRef(node);
+    const bool down = node < kRefDone || inst;
+    const uint64_t m = __ballot(down);
+    if (m == 0 || live - uint32_t(__popcll(m)) + extra >= need) break;
+    if (inst) {
+      instanceStep<STACK>(s.insts, stk, r, rq, FASTONLY || fast, tmax, node, sp);
+    } else if (down) {
+      const float4* np = reinterpret_cast<const float4*>(s.nodes + node);
+      const float4 a = np[0], b = np[1], c = np[2];
+      const uint4 d = reinterpret_cast<const uint4*>(np)[3];
+      nodeStep<STACK, FASTONLY, OCT>(a, b, c, d, stk, r, rq, fast, tmax, node, sp);
+    }
+  }
 }
 
 // Walk the tree for one ray. ANY: stop at the first accepted triangle (returns true).
@@ -381,15 +417,13 @@ __device__ __forceinline__ bool traverse(const DevScene& s, uint32_t* stk, const
   int sp = 0;
   if (!ANY) hit.t = tmax;
   for (;;) {
-    descendRound<STACK, FAST, COUNT, OCT>(s.nodes, s.top, s.topSlots, stk, r, rq, FAST, ANY ? tmax : hit.t, node, sp,
-                                          wait, 0u, cnt.inner);
+    if (ANY && !COUNT && s.twoLevel)
+      descendRoundTwoLevel<STACK, FAST, OCT>(s, stk, r, rq, FAST, tmax, node, sp, wait, 0u);
+    else
+      descendRound<STACK, FAST, COUNT, OCT>(s.nodes, s.top, s.topSlots, stk, r, rq, FAST, ANY ? tmax : hit.t, node,
+                                            sp, wait, 0u, cnt.inner);
     if (node == kRefDone) break;
-    if (node < kRefDone) continue;  // still descending: next round
-    if (ANY && s.twoLevel && node != kRefEmpty && (node & kRefInst)) {
-      node = enterInstance<STACK>(s, stk, sp, node, r, rq, FAST, tmax);
-      if (node == kRefDone) break;
-      continue;
-    }
+    if (node < kRefDone || (ANY && isInstRef(node))) continue;  // still descending: next round
     if (COUNT) cnt.leaves++;
     if (node != kRefEmpty && leafTests<kGridLeafGroup, COUNT>(s.tris, node, r, ANY, tmax, hit, cnt.tests)) return true;
     node = sp > 0 ? stk[(--sp) * kTraceBlock] : kRefDone;
@@ -584,18 +618,24 @@ __device__ __forceinline__ void refillTrace(const DevScene& s, uint32_t* stk, co
       // any hit: a leaf may end the walk, so nothing is loaded ahead (measured: holding records is
       // 1 % slower here)
       uint32_t unused = 0;
-      if (allFast)
+      if (s.twoLevel) {
+        if (allFast)
+          descendRoundTwoLevel<STACK, true>(s, stk, r, rq, fast, lim, node, sp, wait, extra);
+        else
+          descendRoundTwoLevel<STACK, false>(s, stk, r, rq, fast, lim, node, sp, wait, extra);
+      } else if (allFast) {
         descendRound<STACK, true, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
-      else
+      } else {
         descendRound<STACK, false, false>(s.nodes, s.top, s.topSlots, stk, r, rq, fast, lim, node, sp, wait, extra, unused);
+      }
     } else {
       if (allFast)
         descendRoundHeld<STACK, true>(s.nodes, rec, stk, r, rq, fast, lim, node, sp, wait, extra);
       else
         descendRoundHeld<STACK, false>(s.nodes, rec, stk, r, rq, fast, lim, node, sp, wait, extra);
     }
-    if (ANY && s.twoLevel && node > kRefDone && node != kRefEmpty && (node & kRefInst)) {
-      node = enterInstance<STACK>(s, stk, sp, node, r, rq, fast, tmax);
+    if (ANY && isInstRef(node)) {
+      // still at a top-level leaf's instances when the round ended: the next round goes on
     } else if (node > kRefDone) {
       // closest hit: the node after this leaf is the stack top (the walk always goes on), and its record
       // is loaded while the leaf is tested

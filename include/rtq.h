@@ -289,7 +289,7 @@ typedef struct rtq_batch_params {
   uint32_t stack_depth;   /* per-lane LDS stack entries: 24, 32 or 64, >= the deepest scene */
   uint32_t use_graph;     /* 1: replay a captured hipGraph (needs a non-NULL stream) */
   uint32_t leaf_wait;     /* lanes ready for leaf tests that end a round of descent (RTQ_OPTION_LEAF_WAIT);
-                             0 = default */
+                             0 = default: 40 with lane refill, 24 with chunks */
 } rtq_batch_params;
 
 rtq_status rtq_batch_create(rtq_device* dev, rtq_scene* const* scenes, uint32_t num_scenes,

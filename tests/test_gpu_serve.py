@@ -180,6 +180,21 @@ def test_serve_device_requests_equal_alone():
         srv.submit_device(0, B.RTQ_TRACE_ANY, D.DeviceView(d_r, 8, 32 * 10), 10, d_o)
     with pytest.raises(ValueError):
         srv.submit_device(0, B.RTQ_TRACE_CLOSEST, d_r, reqs[1][2].size, D.DeviceView(d_o, 0, 16))
+    # the library checks the buffers itself: host memory, and a range running past its allocation's end
+    import ctypes as C
+    lib, h = dev.lib, C.c_void_p()
+    hb = dev.host_alloc(4096)
+    small = dev.alloc(4096)   # 128 rays
+    big_out = dev.alloc(4 << 20)
+    for rays_ptr, out_ptr, n in ((hb.ptr, big_out.ptr, 16), (small.ptr, hb.ptr, 16), (small.ptr, big_out.ptr, 100_000),
+                                 (small.ptr + 32 * 60, big_out.ptr, 100_000)):
+        st = lib.rtq_server_submit_device(srv.handle, 0, B.RTQ_TRACE_ANY, rays_ptr, n, out_ptr, None, C.byref(h))
+        assert st == 1, (rays_ptr, out_ptr, n, st)  # RTQ_ERR_INVALID_ARGUMENT, nothing queued
+    # the allocation's 128 rays exactly fit
+    small.write(reqs[1][2][:128])
+    req = srv.submit_device(0, B.RTQ_TRACE_ANY, small, 128, D.DeviceView(big_out, 0, 4 * 128))
+    assert req.wait() == B.RTQ_OK
+    req.release()
     srv.close()
 
 

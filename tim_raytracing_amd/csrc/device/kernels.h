@@ -214,6 +214,10 @@ __device__ __forceinline__ bool leafTests(const DevTri* __restrict__ tris, uint3
 // plain order (every lane reaches its leaf before any leaf is tested).
 constexpr uint32_t kGridWait = 24;    // default when each lane traces one ray
 constexpr uint32_t kRefillWait = 32;  // default when lanes refill with new rays
+// default of the lane-refilling batch kernel, whose served launches mix closest- and any-hit rays of
+// many scenes (served capacity on device requests 2086 / 2134 / 2098 Mrays/s at 32 / 40 / 48; a
+// single-scene any-hit refill walk loses at 40: two-level 917 -> 866)
+constexpr uint32_t kBatchRefillWait = 40;
 
 // Top-of-tree staging: a kernel that traces one scene copies the first s.topSlots node records (the
 // top levels, laid out breadth first) into LDS once per block and reads them there. Every thread of

@@ -258,7 +258,8 @@ def test_serve_64_scenes_poisson_full_size():
     """The served form at its stated size: 64 resident scenes of LogU[10K, 3M] triangles and 240
     Poisson-timed requests of LogU[1K, 2M] rays (half closest, half any hit) submitted by 4 client
     threads under a 2 GiB budget. Every ray of every served request equals the request traced alone (one
-    plain dispatch), and the oracle agrees on 4096 strided rays of every request."""
+    plain dispatch), also when the same requests come as device-buffer requests, and the oracle agrees on
+    4096 strided rays of every request."""
     print("case: 64 resident scenes (LogU[10K, 3M] tris), 240 Poisson requests of LogU[1K, 2M] rays from "
           "4 threads, served vs alone vs oracle samples", flush=True)
     nreq = 240
@@ -314,6 +315,39 @@ def test_serve_64_scenes_poisson_full_size():
     assert len(batches) < nreq
     for h in handles:
         h.release()
+    # the same stream as device-buffer requests (rays and results in HBM, traced in place), all submitted
+    # at once from 4 threads onto the same server: every ray again equals the request traced alone
+    bufs = []
+    for si, flags, rays in reqs:
+        closest = flags == B.RTQ_TRACE_CLOSEST
+        bufs.append((dev.upload(rays), dev.alloc((16 if closest else 4) * rays.size),
+                     dev.alloc(4 * rays.size) if closest else None))
+    handles = [None] * nreq
+
+    def dclient(t):
+        for k in range(t, nreq, 4):
+            si, flags, rays = reqs[k]
+            d_r, d_o, d_n = bufs[k]
+            handles[k] = srv.submit_device(si, flags, d_r, rays.size, d_o, d_n)
+
+    th = [threading.Thread(target=dclient, args=(t,)) for t in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    for k, h in enumerate(handles):
+        assert h.wait() == B.RTQ_OK
+        si, flags, rays = reqs[k]
+        _, d_o, d_n = bufs[k]
+        a_out, a_nid = alone[k]
+        if flags == B.RTQ_TRACE_CLOSEST:
+            assert np.array_equal(d_o.read(np.uint32, 4 * rays.size), a_out), f"device request {k}"
+            assert np.array_equal(d_n.read(np.uint32, rays.size), a_nid), f"device request {k} nid"
+        else:
+            assert np.array_equal(d_o.read(np.uint32, rays.size), a_out), f"device request {k}"
+        h.release()
+    print(f"{nreq} device-buffer requests equal alone on every ray", flush=True)
+    del bufs
     srv.close()
     # the oracle on strided samples of every request, one call per scene and mode
     by_scene = {}

@@ -112,7 +112,8 @@ rtq_status rtq_device_destroy(rtq_device* dev);
  *       after each leaf (best for incoherent rays such as diffuse bounces);
  *   2 = a fixed grid in which each wave owns a contiguous range of rays and refills its lanes from
  *       it (lanes keep neighbouring, coherent rays).
- * RTQ_OPTION_BLOCKS_PER_CU: persistent blocks of 128 lanes per compute unit (1..16, default 8). */
+ * RTQ_OPTION_BLOCKS_PER_CU: persistent blocks of 128 lanes per compute unit (1..16), or 0 (default) =
+ *   automatic: 8, and 12 for two-level scenes (as many as the compute unit's LDS holds). */
 #define RTQ_OPTION_PERSISTENT 1u
 #define RTQ_OPTION_BLOCKS_PER_CU 2u
 /* RTQ_OPTION_SORT_RAYS: 1 = rtq_dispatch_rays first sorts the rays by a Morton code of the origin over
@@ -125,7 +126,7 @@ rtq_status rtq_device_destroy(rtq_device* dev);
 /* RTQ_OPTION_LEAF_WAIT: a wave interleaves descent toward the next leaf with the tests of the leaves
  * its lanes have reached; a round of descent ends once this many of its 64 lanes are ready for leaf
  * tests (1..64; 64 = every lane reaches its leaf first). 0 (default) = 24 for one ray per lane, 32 for
- * refilling dispatches. Results are identical for every value. */
+ * refilling dispatches (24 for those of two-level scenes). Results are identical for every value. */
 #define RTQ_OPTION_LEAF_WAIT 4u
 /* Options apply to dispatches issued after the call; set them while no other thread dispatches on the
  * device (they are plain fields, read when a dispatch is enqueued). */

@@ -275,6 +275,14 @@ def test_leaf_wait_option_bounds():
     assert d.get_option(B.RTQ_OPTION_LEAF_WAIT) == 0
 
 
+def test_blocks_per_cu_option_bounds():
+    d = G.dev()
+    assert d.get_option(B.RTQ_OPTION_BLOCKS_PER_CU) == 0  # automatic
+    with pytest.raises(Exception):
+        d.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, 17)
+    assert d.get_option(B.RTQ_OPTION_BLOCKS_PER_CU) == 0
+
+
 def test_box_scene_any_hit():
     _print("case: closed box any-hit, random t limits")
     s = G.scene(1)

@@ -218,6 +218,10 @@ constexpr uint32_t kRefillWait = 32;  // default when lanes refill with new rays
 // many scenes (served capacity on device requests 2086 / 2134 / 2098 Mrays/s at 32 / 40 / 48; a
 // single-scene any-hit refill walk loses at 40: two-level 917 -> 866)
 constexpr uint32_t kBatchRefillWait = 40;
+// default of a lane-refilling any-hit walk of a two-level scene, whose rays end at very different
+// times (1024 BLAS instances, 4 Mi rays: 946 / 949 / 942 Mrays/s at 20 / 24 / 28, 928 at 32, with 8
+// blocks per CU)
+constexpr uint32_t kTwoLevelRefillWait = 24;
 
 // Top-of-tree staging: a kernel that traces one scene copies the first s.topSlots node records (the
 // top levels, laid out breadth first) into LDS once per block and reads them there. Every thread of

@@ -26,7 +26,7 @@ struct rtq_device {
   int ordinal = 0;
   int numCU = 0;
   uint32_t persistent = 0;
-  uint32_t blocksPerCU = 8;
+  uint32_t blocksPerCU = 0;  // 0 = automatic (dispatch picks 8, or 12 for two-level scenes)
   uint32_t sortRays = 0;
   uint32_t leafWait = 0;  // 0: rtq::kGridWait for grid dispatches, rtq::kRefillWait for refilling ones
   // ray-sort workspace (keys, values and rocPRIM's temporary storage), grown on demand and shared by
@@ -207,7 +207,7 @@ rtq_status rtq_device_set_option(rtq_device* dev, uint32_t option, uint32_t valu
       dev->persistent = value;
       return RTQ_OK;
     case RTQ_OPTION_BLOCKS_PER_CU:
-      if (value < 1 || value > 16) return fail(RTQ_ERR_INVALID_ARGUMENT, "blocks per CU in 1..16");
+      if (value > 16) return fail(RTQ_ERR_INVALID_ARGUMENT, "blocks per CU is 0 (automatic) or 1..16");
       dev->blocksPerCU = value;
       return RTQ_OK;
     case RTQ_OPTION_SORT_RAYS:
@@ -487,7 +487,14 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   // two-level scenes take the any-hit kernels below, whose leaf phase enters BLAS instances
   if (s->view.twoLevel && (flags != RTQ_TRACE_ANY || counters)) return twoLevelOnlyAny();
   const uint32_t gridWait = s->dev->leafWait ? s->dev->leafWait : rtq::kGridWait;
-  const uint32_t refillWait = s->dev->leafWait ? s->dev->leafWait : rtq::kRefillWait;
+  const uint32_t refillWait = s->dev->leafWait      ? s->dev->leafWait
+                              : s->view.twoLevel    ? rtq::kTwoLevelRefillWait
+                                                    : rtq::kRefillWait;
+  // persistent blocks per CU: the option, or automatically 8 (more waves of incoherent one-level walks
+  // thrash the L2: soup bounces 602 / 600 / 561 Mrays/s at 7 / 8 / 10) and 12 for two-level scenes,
+  // whose any-hit walks keep a quarter of their lanes busy and gain from every wave that fits (LDS
+  // holds 10 blocks with the 32-entry stack: 949 / 972 / 978 Mrays/s at 8 / 10 / 12)
+  const uint32_t blocksPerCU = s->dev->blocksPerCU ? s->dev->blocksPerCU : (s->view.twoLevel ? 12u : 8u);
   const uint32_t* perm = nullptr;
   std::unique_lock<std::mutex> sortLock;
   if (!counters && s->dev->sortRays) {
@@ -495,7 +502,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
     if (ss != RTQ_OK) return ss;
   }
   if (!counters && s->dev->persistent == 2) {
-    const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
+    const uint64_t lanes = uint64_t(s->dev->numCU) * blocksPerCU * rtq::kTraceBlock;
     const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
     RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
       constexpr int K = decltype(S)::value;
@@ -512,7 +519,7 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   if (!counters && s->dev->persistent) {
     uint32_t* counter = s->counters + rtq::kXcds * (s->nextCounter.fetch_add(1) % kCounterRing);
     hipLaunchKernelGGL(rtq::kResetCounter, dim3(1), dim3(64), 0, st, counter);
-    const uint64_t lanes = uint64_t(s->dev->numCU) * s->dev->blocksPerCU * rtq::kTraceBlock;
+    const uint64_t lanes = uint64_t(s->dev->numCU) * blocksPerCU * rtq::kTraceBlock;
     const dim3 pgrid(uint32_t(std::min<uint64_t>(lanes, uint64_t(n) + rtq::kTraceBlock - 1) / rtq::kTraceBlock));
     RTQ_WITH_STACK(s->info.max_depth, [&](auto S) {
       constexpr int K = decltype(S)::value;

@@ -28,7 +28,7 @@ struct rtq_device {
   uint32_t persistent = 0;
   uint32_t blocksPerCU = 0;  // 0 = automatic (dispatch picks 8, or 12 for two-level scenes)
   uint32_t sortRays = 0;
-  uint32_t leafWait = 0;  // 0: rtq::kGridWait for grid dispatches, rtq::kRefillWait for refilling ones
+  uint32_t leafWait = 0;  // 0: the kernels' defaults (rtq::kGridWait, kGridAnyWait, kRefillWait, ...)
   // ray-sort workspace (keys, values and rocPRIM's temporary storage), grown on demand and shared by
   // every sorting dispatch on this device; `sortDone` marks the end of the last dispatch that read it
   std::mutex sortMu;
@@ -486,7 +486,9 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   const float4* r = reinterpret_cast<const float4*>(rays);
   // two-level scenes take the any-hit kernels below, whose leaf phase enters BLAS instances
   if (s->view.twoLevel && (flags != RTQ_TRACE_ANY || counters)) return twoLevelOnlyAny();
-  const uint32_t gridWait = s->dev->leafWait ? s->dev->leafWait : rtq::kGridWait;
+  const uint32_t gridWait = s->dev->leafWait          ? s->dev->leafWait
+                            : flags == RTQ_TRACE_ANY ? rtq::kGridAnyWait
+                                                     : rtq::kGridWait;
   const uint32_t refillWait = s->dev->leafWait      ? s->dev->leafWait
                               : s->view.twoLevel    ? rtq::kTwoLevelRefillWait
                                                     : rtq::kRefillWait;

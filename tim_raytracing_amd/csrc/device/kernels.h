@@ -542,8 +542,15 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceAny(DevScene s0, const floa
 //     (takeChunk) when the wave's current chunk runs out (one atomic per chunk, so waves balance
 //     dynamically);
 //   RANGE = true: one fixed contiguous range per wave (no atomics).
-// Either way a wave's lanes hold nearby rays of the buffer.
-constexpr uint32_t kRefillChunk = 256;
+// Either way a wave's lanes hold nearby rays of the buffer. Chunk size, measured (soup-3M bounce 1 /
+// bounce 2 and two-level any hit, Mrays/s): 16: 608 / 849, 806; 32: 622 / 915, 979; 64: 625 / 947,
+// 1048; 128: 607 / 945, 1040; 256: 572 / 907, 966; 512: 495 / 810, 875. Smaller chunks balance the
+// waves' last chunks and the moves to other XCDs' shares at a finer grain, down to one chunk filling a
+// whole wave.
+constexpr uint32_t kRefillChunk = 64;
+// chunk of the lane-refilling batch kernel (kBatchRefill), whose launches hold many items of many
+// scenes: served capacity on device-buffer requests 2075 / 2122 / 2133-2139 Mrays/s at 64 / 128 / 256
+constexpr uint32_t kBatchRefillChunk = 256;
 
 // Chunk queues, one per XCD. Blocks are handed to the 8 XCDs round-robin (block b runs on XCD b % 8), and
 // each XCD has its own L2. A wave first takes chunks from its own XCD's share of the rays, a contiguous
@@ -1063,7 +1070,7 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchPersistent(const DevScene* 
 }
 
 // Lane-refilling form of the batch: lanes take single rays from the concatenation of all items (the
-// wave draws chunks of kRefillChunk rays from the per-XCD chunk queues, so an XCD works through the
+// wave draws chunks of kBatchRefillChunk rays from the per-XCD chunk queues, so an XCD works through the
 // items of its own share first; a binary search over rayBegin maps a ray to its item), so a lane that
 // finishes early picks up the next ray, whatever its item, scene or mode, instead of idling until its
 // wave's slowest ray is done. The scene pointers and the closest/any mode are per lane. Each ray still
@@ -1090,7 +1097,7 @@ __global__ __launch_bounds__(kTraceBlock) void kBatchRefill(const BatchItemDev* 
   NodeRec rec{};
   for (;;) {
     const uint64_t idle = __ballot(!active);
-    if (idle && !exhausted && next >= end && !takeChunk(counter, totalRays, kRefillChunk, q, tried, next, end))
+    if (idle && !exhausted && next >= end && !takeChunk(counter, totalRays, kBatchRefillChunk, q, tried, next, end))
       exhausted = true;
     if (idle && next < end) {
       const uint32_t want = uint32_t(__popcll(idle));

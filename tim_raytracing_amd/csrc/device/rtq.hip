@@ -492,10 +492,11 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   const uint32_t refillWait = s->dev->leafWait      ? s->dev->leafWait
                               : s->view.twoLevel    ? rtq::kTwoLevelRefillWait
                                                     : rtq::kRefillWait;
-  // persistent blocks per CU: the option, or automatically 8 (more waves of incoherent one-level walks
-  // thrash the L2: soup bounces 602 / 600 / 561 Mrays/s at 7 / 8 / 10) and 12 for two-level scenes,
-  // whose any-hit walks keep a quarter of their lanes busy and gain from every wave that fits (LDS
-  // holds 10 blocks with the 32-entry stack: 949 / 972 / 978 Mrays/s at 8 / 10 / 12)
+  // persistent blocks per CU: the option, or automatically 8 for one-level scenes (soup bounces: 757 /
+  // 750 / 752 Mrays/s at 7 / 8 / 10 blocks with 64-ray chunks; with 256-ray chunks, whose wider span of
+  // rays in flight crowded the L2, 602 / 600 / 561) and 12 for two-level scenes, whose any-hit walks
+  // keep a quarter of their lanes busy and gain from every wave that fits (LDS holds 10 blocks with the
+  // 32-entry stack: 949 / 972 / 978 Mrays/s at 8 / 10 / 12)
   const uint32_t blocksPerCU = s->dev->blocksPerCU ? s->dev->blocksPerCU : (s->view.twoLevel ? 12u : 8u);
   const uint32_t* perm = nullptr;
   std::unique_lock<std::mutex> sortLock;

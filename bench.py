@@ -117,7 +117,7 @@ def configure_dispatch(dev, args, auto: str, auto_sort: int = 0):
     dev.set_option(B.RTQ_OPTION_LEAF_WAIT, args.leaf_wait)
     if args.blocks_per_cu:
         dev.set_option(B.RTQ_OPTION_BLOCKS_PER_CU, args.blocks_per_cu)
-    return {"mode": args.dispatch, "blocks_per_cu": dev.get_option(B.RTQ_OPTION_BLOCKS_PER_CU) or "auto (8; 12 two-level)",
+    return {"mode": args.dispatch, "blocks_per_cu": dev.get_option(B.RTQ_OPTION_BLOCKS_PER_CU) or "auto (7; 12 two-level)",
             "sort_rays": args.sort,
             "leaf_wait": args.leaf_wait}
 

@@ -113,7 +113,7 @@ rtq_status rtq_device_destroy(rtq_device* dev);
  *   2 = a fixed grid in which each wave owns a contiguous range of rays and refills its lanes from
  *       it (lanes keep neighbouring, coherent rays).
  * RTQ_OPTION_BLOCKS_PER_CU: persistent blocks of 128 lanes per compute unit (1..16), or 0 (default) =
- *   automatic: 8, and 12 for two-level scenes (as many as the compute unit's LDS holds). */
+ *   automatic: 7, and 12 for two-level scenes (as many as the compute unit's LDS holds). */
 #define RTQ_OPTION_PERSISTENT 1u
 #define RTQ_OPTION_BLOCKS_PER_CU 2u
 /* RTQ_OPTION_SORT_RAYS: 1 = rtq_dispatch_rays first sorts the rays by a Morton code of the origin over

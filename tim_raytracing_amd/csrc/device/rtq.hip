@@ -26,7 +26,7 @@ struct rtq_device {
   int ordinal = 0;
   int numCU = 0;
   uint32_t persistent = 0;
-  uint32_t blocksPerCU = 0;  // 0 = automatic (dispatch picks 8, or 12 for two-level scenes)
+  uint32_t blocksPerCU = 0;  // 0 = automatic (dispatch picks 7, or 12 for two-level scenes)
   uint32_t sortRays = 0;
   uint32_t leafWait = 0;  // 0: the kernels' defaults (rtq::kGridWait, kGridAnyWait, kRefillWait, ...)
   // ray-sort workspace (keys, values and rocPRIM's temporary storage), grown on demand and shared by
@@ -492,12 +492,13 @@ static rtq_status dispatch(rtq_scene* s, const rtq_ray* rays, uint32_t n, uint32
   const uint32_t refillWait = s->dev->leafWait      ? s->dev->leafWait
                               : s->view.twoLevel    ? rtq::kTwoLevelRefillWait
                                                     : rtq::kRefillWait;
-  // persistent blocks per CU: the option, or automatically 8 for one-level scenes (soup bounces: 757 /
-  // 750 / 752 Mrays/s at 7 / 8 / 10 blocks with 64-ray chunks; with 256-ray chunks, whose wider span of
-  // rays in flight crowded the L2, 602 / 600 / 561) and 12 for two-level scenes, whose any-hit walks
+  // persistent blocks per CU: the option, or automatically 7 for one-level scenes (soup bounces with
+  // 64-ray chunks: 727 / 753 / 752 / 752 Mrays/s at 6 / 7 / 8 / 10 blocks, and 4 % fewer bytes beyond the
+  // L2 per ray at 7 than at 8, the span of rays in flight being smaller; with 256-ray chunks, whose wider
+  // span crowded the L2, 602 / 600 / 561 at 7 / 8 / 10) and 12 for two-level scenes, whose any-hit walks
   // keep a quarter of their lanes busy and gain from every wave that fits (LDS holds 10 blocks with the
   // 32-entry stack: 949 / 972 / 978 Mrays/s at 8 / 10 / 12)
-  const uint32_t blocksPerCU = s->dev->blocksPerCU ? s->dev->blocksPerCU : (s->view.twoLevel ? 12u : 8u);
+  const uint32_t blocksPerCU = s->dev->blocksPerCU ? s->dev->blocksPerCU : (s->view.twoLevel ? 12u : 7u);
   const uint32_t* perm = nullptr;
   std::unique_lock<std::mutex> sortLock;
   if (!counters && s->dev->sortRays) {

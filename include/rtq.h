@@ -124,11 +124,11 @@ rtq_status rtq_device_destroy(rtq_device* dev);
  * previous one's traversal to finish), so they are safe from any streams but do not overlap. Default 0. */
 #define RTQ_OPTION_SORT_RAYS 3u
 /* RTQ_OPTION_LEAF_WAIT: a wave interleaves descent toward the next leaf with the tests of the leaves
- * its lanes have reached; a round of descent ends once this many of its 64 lanes are ready for leaf
- * tests (1..64; 64 = every lane reaches its leaf first). 0 (default) = 24 for one closest-hit ray per
- * lane, 8 for one any-hit ray per lane (the value the shadow, record and probe passes always use for
- * their any-hit rays), 32 for refilling dispatches (24 for those of two-level scenes). Results are
- * identical for every value. */
+ * its lanes have reached; a round of descent ends once this many lanes per 64 of the wave's live lanes
+ * are ready for leaf tests (1..64; 64 = every lane reaches its leaf first). 0 (default) = 24 for one
+ * closest-hit ray per lane, 16 for one any-hit ray per lane (the value the shadow, record and probe
+ * passes always use for their any-hit rays), 32 for refilling dispatches (24 for those of two-level
+ * scenes). Results are identical for every value. */
 #define RTQ_OPTION_LEAF_WAIT 4u
 /* Options apply to dispatches issued after the call; set them while no other thread dispatches on the
  * device (they are plain fields, read when a dispatch is enqueued). */

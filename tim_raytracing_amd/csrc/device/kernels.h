@@ -206,18 +206,21 @@ __device__ __forceinline__ bool leafTests(const DevTri* __restrict__ tris, uint3
 // A wave alternates between descent (each lane steps down inner nodes toward its next leaf) and a
 // leaf phase (lanes at a leaf test its triangles). If every descent ran until all lanes had reached a
 // leaf, the lanes that got there first would idle while the deepest one descends, and on incoherent
-// rays most instructions would run with a few lanes on. So a round of descent ends as soon as `wait`
-// lanes of the wave are ready for the leaf phase (at a leaf or finished, plus `extra` lanes that are
-// ready anyway, such as free lanes about to take a new ray); lanes still at an inner node keep their
-// node and stack and go on in the next round. Each ray's own sequence of operations, and so every
+// rays most instructions would run with a few lanes on. So a round of descent ends as soon as the
+// lanes ready for the leaf phase (at a leaf or finished, plus `extra` lanes that are ready anyway, such
+// as free lanes about to take a new ray) reach `wait` per 64 of the wave's live and extra lanes: a wave
+// whose other lanes have ended (hit lanes only in a shadow pass, the last rays of a refilling launch)
+// keeps the same fraction instead of running every lane to its leaf. Lanes still at an inner node keep
+// their node and stack and go on in the next round. Each ray's own sequence of operations, and so every
 // result, is the same for any `wait`; only the interleaving of the lanes changes. wait = 64 gives the
-// plain order (every lane reaches its leaf before any leaf is tested).
+// plain order (every lane reaches its leaf before any leaf is tested). The threshold scaled to the live
+// lanes instead of a fixed count, measured: soup bounces 754 -> 781 Mrays/s, two-level any hit 1048 ->
+// 1138, probe field 0.63 -> 0.57 ms, 1080p primary and shadow passes unchanged.
 constexpr uint32_t kGridWait = 24;    // default when each lane traces one closest-hit ray
-// default when each lane traces one any-hit ray (shadow rays, probe segments, any-hit grid dispatches):
-// these walks end at their first hit, so shorter rounds leave fewer lanes waiting at a leaf (1080p
-// sphere frame: shadow pass 0.380 / 0.351 / 0.342 / 0.334 ms and probe field 0.787 / 0.718 / 0.680 /
-// 0.624 ms at 24 / 16 / 12 / 8, while its closest-hit rays lose 4 % at 8)
-constexpr uint32_t kGridAnyWait = 8;
+// default when each lane traces one any-hit ray (shadow rays, probe segments, any-hit grid dispatches),
+// which ends at its first hit (1080p sphere frame: shadow pass 0.345 / 0.336 / 0.361 ms and probe field
+// 0.573 / 0.575 / 0.630 ms at 24 / 16 / 8)
+constexpr uint32_t kGridAnyWait = 16;
 constexpr uint32_t kRefillWait = 32;  // default when lanes refill with new rays
 // default of the lane-refilling batch kernel, whose served launches mix closest- and any-hit rays of
 // many scenes (served capacity on device requests 2086 / 2134 / 2098 Mrays/s at 32 / 40 / 48; a
@@ -289,6 +292,11 @@ __device__ __forceinline__ void nodeStep(const float4 a, const float4 b, const f
   }
 }
 
+// Ready lanes that end a round: `wait` per 64 of the wave's `lanes` (live plus extra), at least one.
+__device__ __forceinline__ uint32_t roundNeed(uint32_t wait, uint32_t lanes) {
+  return max(1u, (wait * lanes + 63u) / 64u);
+}
+
 // One round of descent for the calling lanes (see nodeStep for FASTONLY and OCT).
 template <int STACK, bool FASTONLY, bool COUNT, int OCT = -1>
 __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, const DevNode* top, uint32_t topSlots,
@@ -296,7 +304,7 @@ __device__ __forceinline__ void descendRound(const DevNode* __restrict__ nodes, 
                                              uint32_t& node, int& sp, uint32_t wait, uint32_t extra,
                                              uint32_t& inner) {
   const uint32_t live = uint32_t(__popcll(__ballot(1)));
-  const uint32_t need = wait ? wait : 1u;  // a round that ends, ends with a lane ready
+  const uint32_t need = roundNeed(wait, live + extra);
   for (;;) {
     const bool down = node < kRefDone;
     const uint64_t m = __ballot(down);
@@ -338,7 +346,7 @@ __device__ __forceinline__ void descendRoundHeld(const DevNode* __restrict__ nod
                                                  const Ray3& r, const RayRcp& rq, bool fast, float lim,
                                                  uint32_t& node, int& sp, uint32_t wait, uint32_t extra) {
   const uint32_t live = uint32_t(__popcll(__ballot(1)));
-  const uint32_t need = wait ? wait : 1u;
+  const uint32_t need = roundNeed(wait, live + extra);
   for (;;) {
     const bool down = node < kRefDone;
     const uint64_t m = __ballot(down);
@@ -402,7 +410,7 @@ __device__ __forceinline__ void descendRoundTwoLevel(const DevScene& s, uint32_t
                                                      const RayRcp& rq, bool fast, float tmax, uint32_t& node, int& sp,
                                                      uint32_t wait, uint32_t extra) {
   const uint32_t live = uint32_t(__popcll(__ballot(1)));
-  const uint32_t need = wait ? wait : 1u;
+  const uint32_t need = roundNeed(wait, live + extra);
   for (;;) {
     const bool inst = isInstRef(node);
     const bool down = node < kRefDone || inst;

@@ -127,7 +127,7 @@ rtq_status rtq_device_destroy(rtq_device* dev);
  * its lanes have reached; a round of descent ends once this many lanes per 64 of the wave's live lanes
  * are ready for leaf tests (1..64; 64 = every lane reaches its leaf first). 0 (default) = 24 for one
  * closest-hit ray per lane, 16 for one any-hit ray per lane (the value the shadow, record and probe
- * passes always use for their any-hit rays), 32 for refilling dispatches (24 for those of two-level
+ * passes always use for their any-hit rays), 40 for refilling dispatches (24 for those of two-level
  * scenes). Results are identical for every value. */
 #define RTQ_OPTION_LEAF_WAIT 4u
 /* Options apply to dispatches issued after the call; set them while no other thread dispatches on the

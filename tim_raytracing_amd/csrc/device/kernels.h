@@ -221,14 +221,15 @@ constexpr uint32_t kGridWait = 24;    // default when each lane traces one close
 // which ends at its first hit (1080p sphere frame: shadow pass 0.345 / 0.336 / 0.361 ms and probe field
 // 0.573 / 0.575 / 0.630 ms at 24 / 16 / 8)
 constexpr uint32_t kGridAnyWait = 16;
-constexpr uint32_t kRefillWait = 32;  // default when lanes refill with new rays
+// default when lanes refill with new rays (soup bounces 748 / 780 / 790 / 773 Mrays/s at 24 / 32 / 40 /
+// 48 with the threshold scaled to the live lanes)
+constexpr uint32_t kRefillWait = 40;
 // default of the lane-refilling batch kernel, whose served launches mix closest- and any-hit rays of
 // many scenes (served capacity on device requests 2086 / 2134 / 2098 Mrays/s at 32 / 40 / 48; a
 // single-scene any-hit refill walk loses at 40: two-level 917 -> 866)
 constexpr uint32_t kBatchRefillWait = 40;
 // default of a lane-refilling any-hit walk of a two-level scene, whose rays end at very different
-// times (1024 BLAS instances, 4 Mi rays: 946 / 949 / 942 Mrays/s at 20 / 24 / 28, 928 at 32, with 8
-// blocks per CU)
+// times (1024 BLAS instances, 4 Mi rays: 1083 / 1141 / 1141 Mrays/s at 16 / 24 / 32)
 constexpr uint32_t kTwoLevelRefillWait = 24;
 
 // Top-of-tree staging: a kernel that traces one scene copies the first s.topSlots node records (the

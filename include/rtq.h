@@ -125,7 +125,7 @@ rtq_status rtq_device_destroy(rtq_device* dev);
 #define RTQ_OPTION_SORT_RAYS 3u
 /* RTQ_OPTION_LEAF_WAIT: a wave interleaves descent toward the next leaf with the tests of the leaves
  * its lanes have reached; a round of descent ends once this many lanes per 64 of the wave's live lanes
- * are ready for leaf tests (1..64; 64 = every lane reaches its leaf first). 0 (default) = 24 for one
+ * are ready for leaf tests (1..64; 64 = every lane reaches its leaf first). 0 (default) = 40 for one
  * closest-hit ray per lane, 16 for one any-hit ray per lane (the value the shadow, record and probe
  * passes always use for their any-hit rays), 40 for refilling dispatches (24 for those of two-level
  * scenes). Results are identical for every value. */
@@ -292,7 +292,7 @@ typedef struct rtq_batch_params {
   uint32_t stack_depth;   /* per-lane LDS stack entries: 24, 32 or 64, >= the deepest scene */
   uint32_t use_graph;     /* 1: replay a captured hipGraph (needs a non-NULL stream) */
   uint32_t leaf_wait;     /* lanes ready for leaf tests that end a round of descent (RTQ_OPTION_LEAF_WAIT);
-                             0 = default: 40 with lane refill, 24 with chunks */
+                             0 = default: 40 */
 } rtq_batch_params;
 
 rtq_status rtq_batch_create(rtq_device* dev, rtq_scene* const* scenes, uint32_t num_scenes,

@@ -216,7 +216,9 @@ __device__ __forceinline__ bool leafTests(const DevTri* __restrict__ tris, uint3
 // plain order (every lane reaches its leaf before any leaf is tested). The threshold scaled to the live
 // lanes instead of a fixed count, measured: soup bounces 754 -> 781 Mrays/s, two-level any hit 1048 ->
 // 1138, probe field 0.63 -> 0.57 ms, 1080p primary and shadow passes unchanged.
-constexpr uint32_t kGridWait = 24;    // default when each lane traces one closest-hit ray
+// default when each lane traces one closest-hit ray (1080p sphere frame, tiled primary rays: 3951 /
+// 3997 / 4006 / 3958 / 3815 / 3665 Mrays/s at 24 / 32 / 40 / 48 / 56 / 64, the means of three runs)
+constexpr uint32_t kGridWait = 40;
 // default when each lane traces one any-hit ray (shadow rays, probe segments, any-hit grid dispatches),
 // which ends at its first hit (1080p sphere frame: shadow pass 0.345 / 0.336 / 0.361 ms and probe field
 // 0.573 / 0.575 / 0.630 ms at 24 / 16 / 8)
